@@ -1,0 +1,134 @@
+/*
+ * qcu_ex.h -- extended C API of the MI355X qcu library (libqcu.so).
+ *
+ * The reference has only the five void entry points of qcu.h; everything here is
+ * new and has no counterpart in the reference.  It adds resident gauge handles,
+ * kappa / c_sw / t boundary, mixed precision, batched right-hand sides, solve
+ * statistics, int error codes and the device-memory helpers that a caller without its
+ * own allocator (tests, benchmarks, ctypes) needs.  Every function returns
+ * QCU_SUCCESS (0) or an error code, and never exits the process.
+ *
+ * Field pointers are HIP device pointers in the qcu.h interface layouts
+ * (complex128): gauge [4][2][Vh][3][3], half spinor [Vh][4][3], full spinor
+ * [2][Vh][4][3], clover [2][Vh][2][6][6].
+ */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "qcu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QCU_SUCCESS = 0,
+  QCU_ERROR_INVALID_VALUE = 1,
+  QCU_ERROR_HIP = 2,
+  QCU_ERROR_OUT_OF_MEMORY = 3,
+  QCU_ERROR_NOT_CONVERGED = 4,
+  QCU_ERROR_BREAKDOWN = 5,
+  QCU_ERROR_NOT_FINITE = 6,
+  QCU_ERROR_STATE = 7,
+  QCU_ERROR_ADMISSION = 8
+};
+
+enum { QCU_PREC_DOUBLE = 0, QCU_PREC_SINGLE = 1, QCU_PREC_HALF = 2 };
+
+typedef struct qcuGauge_st* qcuGauge;
+
+typedef struct {
+  double kappa;    /* hopping parameter */
+  double csw;      /* clover coefficient; 0 = Wilson (A = 1) */
+  int t_boundary;  /* +1 periodic, -1 antiperiodic in t */
+  int recon;       /* link storage: 12 (two rows, third rebuilt) or 18 */
+} qcuOperator;
+
+typedef struct {
+  double tol;      /* stop when ||b - M x|| / ||b|| <= tol (FP64, full lattice) */
+  int max_iter;    /* iteration cap (<= 0: 10 * volume) */
+  int precision;   /* inner precision: QCU_PREC_DOUBLE = plain FP64 CG */
+  double delta;    /* reliable update when |r| < delta * max |r| since the last one */
+  int graph;       /* 1: replay captured hipGraphs of `chunk` iterations */
+  int chunk;       /* iterations between host convergence checks */
+} qcuSolveParams;
+
+typedef struct {
+  int iterations;        /* CG iterations (max over right-hand sides in a batch) */
+  int reliable_updates;  /* FP64 residual recomputations */
+  int converged;         /* 1 if every right-hand side reached tol */
+  int breakdown;         /* 1 if CG met p^dag A p <= 0 or a non-finite value */
+  double true_residual;  /* max over right-hand sides of ||b - M x|| / ||b||, FP64 */
+  double seconds;        /* wall time of the solve */
+} qcuSolveStats;
+
+/* ---- library --------------------------------------------------------------- */
+int qcuInit(int device);
+int qcuFinalize(void);
+int qcuGetLastError(void);
+const char* qcuGetLastErrorString(void);
+void qcuClearLastError(void);
+const char* qcuVersion(void);
+
+/* ---- device memory helpers ------------------------------------------------- */
+int qcuMalloc(void** ptr, size_t bytes);
+int qcuFree(void* ptr);
+int qcuMemcpyHtoD(void* dst, const void* src, size_t bytes);
+int qcuMemcpyDtoH(void* dst, const void* src, size_t bytes);
+int qcuMemcpyDtoD(void* dst, const void* src, size_t bytes);
+int qcuMemset(void* dst, int value, size_t bytes);
+int qcuSynchronize(void);
+int qcuMemInfo(size_t* free_bytes, size_t* total_bytes);
+
+/* ---- resident gauge fields ------------------------------------------------- */
+/* copy and pack an interface-layout gauge (device pointer) */
+int qcuGaugeCreate(qcuGauge* g, const void* gauge, const int lattice_size[4]);
+/* seeded near-unit SU(3) gauge: U = SU3-project(exp(i eps H)) per link */
+int qcuGaugeCreateRandom(qcuGauge* g, const int lattice_size[4], uint64_t seed, double eps);
+int qcuGaugeDestroy(qcuGauge g);
+/* interface-layout copy of the gauge into a device buffer */
+int qcuGaugeExport(qcuGauge g, void* gauge);
+/* A = 1 + kappa*csw*T and A^-1, complex128 [2][Vh][2][6][6] (either may be NULL);
+ * *nfallback (may be NULL) = blocks inverted by the pivoted fallback */
+int qcuCloverExport(qcuGauge g, const qcuOperator* op, void* A, void* Ainv, int* nfallback);
+
+/* ---- sources ---------------------------------------------------------------- */
+/* N(0,1) per real of a full spinor of `lattice_size`, element k from counter k */
+int qcuSourceGaussian(void* v, const int lattice_size[4], uint64_t seed);
+
+/* ---- operators (interface layouts) --------------------------------------- */
+/* out(parity) = D^s in(1-parity), s = -1 if dagger; precision selects the kernel
+ * (inputs are rounded to it); recon 12 or 18 */
+int qcuHop(qcuGauge g, void* out, const void* in, int parity, int dagger, int t_boundary, int recon,
+           int precision);
+/* full lattice: out = (A - kappa D_s) in */
+int qcuApplyM(qcuGauge g, const qcuOperator* op, void* out, const void* in, int dagger);
+/* even half spinors: out = Mhat in = A_ee in - kappa^2 D_eo A_oo^-1 D_oe in (s = +1 or
+ * -1 if dagger), computed in `precision` */
+int qcuApplyMhat(qcuGauge g, const qcuOperator* op, void* out, const void* in, int dagger, int precision);
+
+/* ---- solver ----------------------------------------------------------------- */
+void qcuDefaultSolveParams(qcuSolveParams* sp);
+/* parameters cg_inverter uses */
+int qcuSetDefaultSolve(const qcuSolveParams* sp);
+/* x <- M^-1 b for one full spinor */
+int qcuSolve(qcuGauge g, const qcuOperator* op, const qcuSolveParams* sp, void* x, const void* b,
+             qcuSolveStats* stats);
+/* nrhs full spinors solved together (links loaded once per site for all of them);
+ * stats may be an array of nrhs entries or NULL */
+int qcuSolveBatch(qcuGauge g, const qcuOperator* op, const qcuSolveParams* sp, int nrhs, void* const* x,
+                  const void* const* b, qcuSolveStats* stats);
+
+/* ---- measurement ------------------------------------------------------------ */
+/* Times `iters` back-to-back even-odd hop applications (alternating parities) on
+ * internal-layout fields of `precision`, nrhs right-hand sides per launch, with
+ * hipEvents on the library stream.  ms = average per launch. */
+int qcuBenchHop(qcuGauge g, int precision, int recon, int nrhs, int iters, double* ms);
+/* device copy bandwidth: ms per copy of `bytes` (read + write) */
+int qcuBenchCopy(size_t bytes, int iters, double* ms);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,632 @@
+// api.cpp -- libqcu.so C ABI: the five drop-in entry points of qcu.h and the
+// extended API of qcu_ex.h, on top of the resident-field runtime.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "qcu_host.h"
+
+namespace qcu {
+
+// ------------------------------------------------------------ errors ------
+static int g_last_code = QCU_SUCCESS;
+static std::string g_last_msg;
+
+void check_hip(hipError_t e, const char* what, const char* file, int line) {
+  if (e == hipSuccess) return;
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) from %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  throw Error(e == hipErrorOutOfMemory ? QCU_ERROR_OUT_OF_MEMORY : QCU_ERROR_HIP, buf);
+}
+
+static void record(int code, const std::string& msg) {
+  g_last_code = code;
+  g_last_msg = msg;
+}
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    f();
+    return QCU_SUCCESS;
+  } catch (const Error& e) {
+    record(e.code, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    record(QCU_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    return QCU_ERROR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    record(QCU_ERROR_STATE, e.what());
+    return QCU_ERROR_STATE;
+  }
+}
+
+// drop-in entry points report like the reference's checkCudaErrors: message on
+// stderr, then exit(-1) unless QCU_EXIT_ON_ERROR=0
+template <typename F>
+static void dropin(const char* name, F&& f) {
+  int rc = guarded(f);
+  if (rc == QCU_SUCCESS) return;
+  fprintf(stderr, "qcu: %s failed (error %d): %s\n", name, rc, g_last_msg.c_str());
+  const char* env = getenv("QCU_EXIT_ON_ERROR");
+  if (!(env && strcmp(env, "0") == 0)) exit(-1);
+}
+
+static bool quiet() {
+  const char* e = getenv("QCU_QUIET");
+  return e && strcmp(e, "0") != 0;
+}
+
+// ------------------------------------------------------------ buffers -----
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+void DevBuf::release() {
+  if (p) QHIP(hipFree(p));
+  p = nullptr;
+  n = 0;
+}
+void DevBuf::alloc(size_t bytes) {
+  release();
+  if (bytes == 0) return;
+  QHIP(hipMalloc(&p, bytes));
+  n = bytes;
+}
+void DevBuf::ensure(size_t bytes) {
+  if (bytes > n) alloc(bytes);
+}
+
+Context& ctx() {
+  static Context c;
+  return c;
+}
+
+DevBuf* Context::buf(const std::string& name, size_t bytes) {
+  auto& slot = scratch[name];
+  if (!slot) slot.reset(new DevBuf);
+  if (slot->n < bytes) {
+    QHIP(hipStreamSynchronize(stream));  // the old buffer may still be in use
+    slot->alloc(bytes);
+  }
+  return slot.get();
+}
+
+void ensure_init() {
+  Context& c = ctx();
+  if (c.inited) return;
+  int n = 0;
+  QHIP(hipGetDeviceCount(&n));
+  if (n <= 0) throw Error(QCU_ERROR_HIP, "no HIP device");
+  QHIP(hipSetDevice(c.device));
+  QHIP(hipStreamCreate(&c.stream));  // default priority, blocking w.r.t. the null stream
+  qcuDefaultSolveParams(&c.default_solve);
+  c.default_solve.precision = QCU_PREC_DOUBLE;
+  c.default_solve.tol = 1e-12;
+  c.inited = true;
+}
+
+void validate_lattice(const int L[4]) {
+  for (int i = 0; i < 4; ++i) QREQ(L[i] > 0, "lattice sizes must be positive");
+  QREQ(L[0] % 2 == 0, "Lx must be even");
+  double v = (double)L[0] * L[1] * L[2] * L[3];
+  QREQ(v / 2 < 2147483647.0, "half volume must fit in 31 bits");
+}
+
+// --------------------------------------------------------- gauge fields ---
+size_t GaugeField::bytes() const {
+  size_t s = aos.n;
+  for (auto& a : packed)
+    for (auto& b : a) s += b.n;
+  for (auto& kv : clovers)
+    for (int p = 0; p < 3; ++p) s += kv.second->A[p].n + kv.second->Ainv[p].n;
+  return s;
+}
+
+const void* GaugeField::get(int prec, int recon) {
+  DevBuf& b = packed[prec][recon == 12 ? 1 : 0];
+  if (!b.p) {
+    b.alloc(gauge_bytes(prec, recon, geo.vh));
+    launch_gauge_pack(prec, recon, aos.p, b.p, geo.vh, stream());
+    QHIP(hipGetLastError());
+  }
+  return b.p;
+}
+
+CloverField* GaugeField::clover(double ck, int prec) {
+  auto it = clovers.find(ck);
+  CloverField* cf;
+  if (it == clovers.end()) {
+    std::unique_ptr<CloverField> c(new CloverField);
+    c->ck = ck;
+    c->vh = geo.vh;
+    c->A[F64].alloc(clover_bytes(F64, geo.vh));
+    c->Ainv[F64].alloc(clover_bytes(F64, geo.vh));
+    const void* g18 = get(F64, 18);
+    DevBuf* G = ctx().buf("clover_G", (size_t)6 * 9 * 16 * geo.vh);
+    DevBuf* nb = ctx().buf("clover_nbad", sizeof(int));
+    QHIP(hipMemsetAsync(nb->p, 0, sizeof(int), stream()));
+    for (int p = 0; p < 2; ++p) {
+      launch_clover_leaves(g18, G->p, geo, p, stream());
+      launch_clover_assemble(G->p, const_cast<void*>(c->a(F64, p)), const_cast<void*>(c->ainv(F64, p)), geo.vh, ck,
+                             static_cast<int*>(nb->p), stream());
+    }
+    QHIP(hipGetLastError());
+    int nbad = 0;
+    QHIP(hipMemcpyAsync(&nbad, nb->p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+    c->nbad = nbad;
+    cf = c.get();
+    clovers[ck] = std::move(c);
+  } else {
+    cf = it->second.get();
+  }
+  if (prec != F64 && !cf->A[prec].p) {
+    cf->A[prec].alloc(clover_bytes(prec, geo.vh));
+    cf->Ainv[prec].alloc(clover_bytes(prec, geo.vh));
+    for (int p = 0; p < 2; ++p) {
+      launch_clover_convert(prec, cf->a(F64, p), const_cast<void*>(cf->a(prec, p)), geo.vh, stream());
+      launch_clover_convert(prec, cf->ainv(F64, p), const_cast<void*>(cf->ainv(prec, p)), geo.vh, stream());
+    }
+    QHIP(hipGetLastError());
+  }
+  return cf;
+}
+
+static std::unique_ptr<GaugeField> new_gauge(const int L[4]) {
+  validate_lattice(L);
+  std::unique_ptr<GaugeField> g(new GaugeField);
+  for (int i = 0; i < 4; ++i) g->L[i] = L[i];
+  g->geo = make_geom(L);
+  g->aos.alloc((size_t)8 * g->geo.vh * 9 * 16);
+  return g;
+}
+
+std::unique_ptr<GaugeField> make_gauge_from_aos(const void* aos_device, const int L[4]) {
+  auto g = new_gauge(L);
+  QHIP(hipMemcpyAsync(g->aos.p, aos_device, g->aos.n, hipMemcpyDeviceToDevice, stream()));
+  return g;
+}
+
+std::unique_ptr<GaugeField> make_gauge_random(const int L[4], uint64_t seed, double eps) {
+  auto g = new_gauge(L);
+  launch_gauge_random(g->aos.p, g->geo.vh, seed, eps, stream());
+  QHIP(hipGetLastError());
+  return g;
+}
+
+// ------------------------------------------------------------- tuning -----
+static std::string tune_key(int prec, int recon, long vh, int nrhs) {
+  char b[96];
+  snprintf(b, sizeof b, "hop:%d:%d:%ld:%d", prec, recon, vh, nrhs);
+  return b;
+}
+Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
+  auto it = ctx().tune.find(tune_key(prec, recon, vh, nrhs));
+  if (it != ctx().tune.end()) return it->second;
+  Tune t;
+  if (nrhs > 1) {
+    t.rb = nrhs >= 4 ? 4 : nrhs;
+    t.block = 64 * t.rb;
+  }
+  return t;
+}
+void set_tuned_hop(int prec, int recon, long vh, int nrhs, const Tune& t) {
+  ctx().tune[tune_key(prec, recon, vh, nrhs)] = t;
+}
+
+void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
+         const void* clov, double coef, int nrhs, double* partials, hipStream_t st) {
+  QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
+  HopArgs a;
+  memset(&a, 0, sizeof a);
+  a.in = in.p;
+  a.out = out.p;
+  a.x = x.p;
+  a.in_stride = in.stride;
+  a.out_stride = out.stride;
+  a.x_stride = x.stride;
+  a.gauge = g.get(prec, recon);
+  for (int r = 0; r < nrhs; ++r) {
+    a.clov[r] = clov;
+    a.coef[r] = coef;
+  }
+  a.partials = partials;
+  a.geo = g.geo;
+  a.parity = parity;
+  a.dagger = dagger;
+  a.tbc = tbc;
+  a.nrhs = nrhs;
+  Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
+  a.xcd_remap = t.xcd;
+  launch_hop(prec, recon, epi, a, t.block, t.rb, st);
+}
+
+}  // namespace qcu
+
+using namespace qcu;
+
+// =================================================================== qcu.h ===
+static QcuParam g_param;
+
+static GaugeField& dropin_gauge(void* gauge, const QcuParam* param, bool reload) {
+  Context& c = ctx();
+  QREQ(gauge != nullptr, "gauge pointer is NULL");
+  QREQ(param != nullptr, "QcuParam pointer is NULL");
+  auto it = c.dropin.find(gauge);
+  bool same_geom = it != c.dropin.end() && memcmp(it->second->L, param->lattice_size, sizeof(int) * 4) == 0;
+  if (it == c.dropin.end() || reload || !same_geom) {
+    auto g = make_gauge_from_aos(gauge, param->lattice_size);
+    launch_gauge_reconstruct(g->aos.p, g->geo.vh, stream());
+    QHIP(hipGetLastError());
+    c.dropin[gauge] = std::move(g);
+  }
+  return *c.dropin[gauge];
+}
+
+extern "C" void initGridSize(QcuGrid_t* grid, QcuParam* p_param, void* gauge, void* fermion_in, void* fermion_out) {
+  (void)fermion_in;
+  (void)fermion_out;
+  dropin("initGridSize", [&] {
+    ensure_init();
+    QREQ(grid && p_param, "initGridSize: NULL argument");
+    for (int i = 0; i < 4; ++i)
+      QREQ(grid->grid_size[i] == 1, "initGridSize: this build keeps the lattice on one GPU (grid_size must be 1,1,1,1)");
+    validate_lattice(p_param->lattice_size);
+    Context& c = ctx();
+    memcpy(c.L, p_param->lattice_size, sizeof c.L);
+    c.have_geom = true;
+    g_param = *p_param;
+    if (gauge) dropin_gauge(gauge, p_param, true);
+  });
+}
+
+extern "C" void loadQcuGauge(void* gauge, QcuParam* param) {
+  dropin("loadQcuGauge", [&] {
+    ensure_init();
+    GaugeField& g = dropin_gauge(gauge, param, true);
+    g.get(F64, 12);
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+extern "C" void dslashQcu(void* fermion_out, void* fermion_in, void* gauge, QcuParam* param, int parity) {
+  dropin("dslashQcu", [&] {
+    ensure_init();
+    QREQ(parity == 0 || parity == 1, "dslashQcu: parity must be 0 or 1");
+    QREQ(fermion_out && fermion_in, "dslashQcu: NULL spinor");
+    GaugeField& g = dropin_gauge(gauge, param, false);
+    long vh = g.geo.vh;
+    DevBuf* a = ctx().buf("dslash_in", spinor_bytes(F64, vh));
+    DevBuf* b = ctx().buf("dslash_out", spinor_bytes(F64, vh));
+    launch_spinor_from_aos(F64, fermion_in, 0, vec(a->p), vh, 1, stream());
+    hop(g, F64, 12, EPI_PLAIN, parity, 0, 1.0, vec(a->p), vec(b->p), vec(nullptr), nullptr, 1.0, 1, nullptr, stream());
+    launch_spinor_to_aos(F64, vec(b->p), fermion_out, 0, vh, 1, stream());
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+extern "C" void fullDslashQcu(void* fermion_out, void* fermion_in, void* gauge, QcuParam* param, int dagger_flag) {
+  dropin("fullDslashQcu", [&] {
+    ensure_init();
+    QREQ(fermion_out && fermion_in, "fullDslashQcu: NULL spinor");
+    GaugeField& g = dropin_gauge(gauge, param, false);
+    qcuOperator op = {1.0, 1.0, 1, 12};
+    apply_m_iface(g, op, fermion_out, fermion_in, dagger_flag ? 1 : 0);
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+extern "C" void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuParam* param) {
+  qcuSolveStats st;
+  memset(&st, 0, sizeof st);
+  dropin("cg_inverter", [&] {
+    ensure_init();
+    QREQ(b_vector && x_vector, "cg_inverter: NULL spinor");
+    GaugeField& g = dropin_gauge(gauge, param, false);
+    qcuOperator op = {1.0, 1.0, 1, 12};
+    void* xs[1] = {x_vector};
+    const void* bs[1] = {b_vector};
+    solve_batch(g, op, ctx().default_solve, 1, xs, bs, &st);
+    QHIP(hipStreamSynchronize(stream()));
+  });
+  if (!quiet()) {
+    // the reference's progress lines (src/cg_inverter.cu:330,342-349)
+    if (st.converged) printf("number of iteration %d\ncg success!!!\n", st.iterations);
+    else printf("cg_failed!!!\n");
+    printf("true residual ||b - M x|| / ||b|| = %.3e\n", st.true_residual);
+  }
+}
+
+// ================================================================ qcu_ex.h ===
+extern "C" {
+
+int qcuInit(int device) {
+  return guarded([&] {
+    Context& c = ctx();
+    if (c.inited && c.device != device) throw Error(QCU_ERROR_STATE, "qcuInit: already initialised on another device");
+    c.device = device;
+    ensure_init();
+  });
+}
+
+int qcuFinalize(void) {
+  return guarded([&] {
+    Context& c = ctx();
+    if (!c.inited) return;
+    QHIP(hipStreamSynchronize(c.stream));
+    c.dropin.clear();
+    c.scratch.clear();
+    QHIP(hipStreamDestroy(c.stream));
+    c.stream = nullptr;
+    c.inited = false;
+  });
+}
+
+int qcuGetLastError(void) { return g_last_code; }
+const char* qcuGetLastErrorString(void) { return g_last_msg.c_str(); }
+void qcuClearLastError(void) { record(QCU_SUCCESS, ""); }
+const char* qcuVersion(void) { return "qcu-mi355x 0.1 (gfx950)"; }
+
+int qcuMalloc(void** ptr, size_t bytes) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(ptr, "qcuMalloc: NULL");
+    QHIP(hipMalloc(ptr, bytes));
+  });
+}
+int qcuFree(void* ptr) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipStreamSynchronize(stream()));
+    QHIP(hipFree(ptr));
+  });
+}
+int qcuMemcpyHtoD(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+int qcuMemcpyDtoH(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+int qcuMemcpyDtoD(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+int qcuMemset(void* dst, int value, size_t bytes) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipMemsetAsync(dst, value, bytes, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+int qcuSynchronize(void) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+int qcuMemInfo(size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipMemGetInfo(free_bytes, total_bytes));
+  });
+}
+
+int qcuGaugeCreate(qcuGauge* out, const void* gauge, const int lattice_size[4]) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(out && gauge && lattice_size, "qcuGaugeCreate: NULL argument");
+    auto g = make_gauge_from_aos(gauge, lattice_size);
+    QHIP(hipStreamSynchronize(stream()));
+    *out = reinterpret_cast<qcuGauge>(g.release());
+  });
+}
+int qcuGaugeCreateRandom(qcuGauge* out, const int lattice_size[4], uint64_t seed, double eps) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(out && lattice_size, "qcuGaugeCreateRandom: NULL argument");
+    QREQ(eps >= 0.0, "eps must be >= 0");
+    auto g = make_gauge_random(lattice_size, seed, eps);
+    QHIP(hipStreamSynchronize(stream()));
+    *out = reinterpret_cast<qcuGauge>(g.release());
+  });
+}
+int qcuGaugeDestroy(qcuGauge h) {
+  return guarded([&] {
+    ensure_init();
+    QHIP(hipStreamSynchronize(stream()));
+    delete reinterpret_cast<GaugeField*>(h);
+  });
+}
+int qcuGaugeExport(qcuGauge h, void* gauge) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && gauge, "qcuGaugeExport: NULL argument");
+    GaugeField& g = *reinterpret_cast<GaugeField*>(h);
+    QHIP(hipMemcpyAsync(gauge, g.aos.p, g.aos.n, hipMemcpyDeviceToDevice, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+int qcuCloverExport(qcuGauge h, const qcuOperator* op, void* A, void* Ainv, int* nfallback) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && op, "qcuCloverExport: NULL argument");
+    GaugeField& g = *reinterpret_cast<GaugeField*>(h);
+    CloverField* cf = g.clover(op->kappa * op->csw, F64);
+    size_t half = (size_t)g.geo.vh * 72 * 16;
+    for (int p = 0; p < 2; ++p) {
+      if (A) launch_clover_unpack(cf->a(F64, p), static_cast<char*>(A) + p * half, g.geo.vh, stream());
+      if (Ainv) launch_clover_unpack(cf->ainv(F64, p), static_cast<char*>(Ainv) + p * half, g.geo.vh, stream());
+    }
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(stream()));
+    if (nfallback) *nfallback = cf->nbad;
+  });
+}
+
+int qcuSourceGaussian(void* v, const int L[4], uint64_t seed) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(v && L, "qcuSourceGaussian: NULL argument");
+    validate_lattice(L);
+    long n = (long)L[0] * L[1] * L[2] * L[3] * 12;
+    launch_source_gauss(v, n, seed, stream());
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qcuHop(qcuGauge h, void* out, const void* in, int parity, int dagger, int t_boundary, int recon, int precision) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && out && in, "qcuHop: NULL argument");
+    QREQ(parity == 0 || parity == 1, "parity must be 0 or 1");
+    QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
+    QREQ(precision >= 0 && precision <= 2, "bad precision");
+    QREQ(t_boundary == 1 || t_boundary == -1, "t_boundary must be +1 or -1");
+    GaugeField& g = *reinterpret_cast<GaugeField*>(h);
+    long vh = g.geo.vh;
+    DevBuf* a = ctx().buf("hop_in", spinor_bytes(F64, vh));
+    DevBuf* b = ctx().buf("hop_out", spinor_bytes(F64, vh));
+    launch_spinor_from_aos(precision, in, 0, vec(a->p), vh, 1, stream());
+    hop(g, precision, recon, EPI_PLAIN, parity, dagger ? 1 : 0, (double)t_boundary, vec(a->p), vec(b->p),
+        vec(nullptr), nullptr, 1.0, 1, nullptr, stream());
+    launch_spinor_to_aos(precision, vec(b->p), out, 0, vh, 1, stream());
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qcuApplyM(qcuGauge h, const qcuOperator* op, void* out, const void* in, int dagger) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && op && out && in, "qcuApplyM: NULL argument");
+    apply_m_iface(*reinterpret_cast<GaugeField*>(h), *op, out, in, dagger ? 1 : 0);
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qcuApplyMhat(qcuGauge h, const qcuOperator* op, void* out, const void* in, int dagger, int precision) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && op && out && in, "qcuApplyMhat: NULL argument");
+    QREQ(precision >= 0 && precision <= 2, "bad precision");
+    apply_mhat_iface(*reinterpret_cast<GaugeField*>(h), *op, out, in, dagger ? 1 : 0, precision);
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+void qcuDefaultSolveParams(qcuSolveParams* sp) {
+  if (!sp) return;
+  sp->tol = 1e-10;
+  sp->max_iter = 0;
+  sp->precision = QCU_PREC_SINGLE;
+  sp->delta = 0.1;
+  sp->graph = 1;
+  sp->chunk = 8;
+}
+
+int qcuSetDefaultSolve(const qcuSolveParams* sp) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(sp, "qcuSetDefaultSolve: NULL");
+    ctx().default_solve = *sp;
+  });
+}
+
+int qcuSolve(qcuGauge h, const qcuOperator* op, const qcuSolveParams* sp, void* x, const void* b,
+             qcuSolveStats* stats) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && op && sp && x && b, "qcuSolve: NULL argument");
+    void* xs[1] = {x};
+    const void* bs[1] = {b};
+    solve_batch(*reinterpret_cast<GaugeField*>(h), *op, *sp, 1, xs, bs, stats);
+  });
+}
+
+int qcuSolveBatch(qcuGauge h, const qcuOperator* op, const qcuSolveParams* sp, int nrhs, void* const* x,
+                  const void* const* b, qcuSolveStats* stats) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && op && sp && x && b, "qcuSolveBatch: NULL argument");
+    QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range (1..16)");
+    solve_batch(*reinterpret_cast<GaugeField*>(h), *op, *sp, nrhs, x, b, stats);
+  });
+}
+
+int qcuBenchHop(qcuGauge h, int precision, int recon, int nrhs, int iters, double* ms) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && ms && iters > 0, "qcuBenchHop: bad argument");
+    QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
+    GaugeField& g = *reinterpret_cast<GaugeField*>(h);
+    long vh = g.geo.vh;
+    size_t fb = spinor_bytes(precision, vh);
+    DevBuf* src = ctx().buf("bench_src", (size_t)vh * 192 * nrhs);
+    DevBuf* a = ctx().buf("bench_a", fb * nrhs);
+    DevBuf* b = ctx().buf("bench_b", fb * nrhs);
+    launch_source_gauss(src->p, vh * 12 * nrhs, 12345, stream());
+    launch_spinor_from_aos(precision, src->p, vh * 192, vec(a->p, (long)fb), vh, nrhs, stream());
+    launch_spinor_from_aos(precision, src->p, vh * 192, vec(b->p, (long)fb), vh, nrhs, stream());
+    Vec va = vec(a->p, (long)fb), vb = vec(b->p, (long)fb);
+    for (int w = 0; w < 3; ++w)
+      hop(g, precision, recon, EPI_PLAIN, w & 1, 0, -1.0, (w & 1) ? vb : va, (w & 1) ? va : vb, vec(nullptr), nullptr,
+          1.0, nrhs, nullptr, stream());
+    hipEvent_t e0, e1;
+    QHIP(hipEventCreate(&e0));
+    QHIP(hipEventCreate(&e1));
+    QHIP(hipEventRecord(e0, stream()));
+    for (int i = 0; i < iters; ++i)
+      hop(g, precision, recon, EPI_PLAIN, i & 1, 0, -1.0, (i & 1) ? vb : va, (i & 1) ? va : vb, vec(nullptr), nullptr,
+          1.0, nrhs, nullptr, stream());
+    QHIP(hipEventRecord(e1, stream()));
+    QHIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QHIP(hipEventElapsedTime(&t, e0, e1));
+    QHIP(hipEventDestroy(e0));
+    QHIP(hipEventDestroy(e1));
+    QHIP(hipGetLastError());
+    *ms = (double)t / iters;
+  });
+}
+
+int qcuBenchCopy(size_t bytes, int iters, double* ms) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(ms && iters > 0 && bytes > 0, "qcuBenchCopy: bad argument");
+    DevBuf* a = ctx().buf("copy_a", bytes);
+    DevBuf* b = ctx().buf("copy_b", bytes);
+    QHIP(hipMemsetAsync(a->p, 1, bytes, stream()));
+    QHIP(hipMemcpyAsync(b->p, a->p, bytes, hipMemcpyDeviceToDevice, stream()));
+    hipEvent_t e0, e1;
+    QHIP(hipEventCreate(&e0));
+    QHIP(hipEventCreate(&e1));
+    QHIP(hipEventRecord(e0, stream()));
+    for (int i = 0; i < iters; ++i)
+      QHIP(hipMemcpyAsync((i & 1) ? a->p : b->p, (i & 1) ? b->p : a->p, bytes, hipMemcpyDeviceToDevice, stream()));
+    QHIP(hipEventRecord(e1, stream()));
+    QHIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QHIP(hipEventElapsedTime(&t, e0, e1));
+    QHIP(hipEventDestroy(e0));
+    QHIP(hipEventDestroy(e1));
+    *ms = (double)t / iters;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// k_blas.hip -- level-1 kernels and the CGLS scalar steps.
+//
+// The reference does each BLAS-1 op as its own kernel plus a host round trip per
+// reduction (src/qcu_complex_computation.cu:10-134, src/cg_inverter.cu:181-256).
+// Here reductions are per-64-site wave partials (fixed butterfly order) summed by a
+// one-block kernel in a fixed order, so every result is bitwise reproducible and
+// independent of how right-hand sides are batched, and all scalars stay on the
+// device: a CG iteration needs no host synchronisation.
+#include "qcu_fields.h"
+#include "qcu_kernels.h"
+
+namespace qcu {
+
+static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// write the wave's partial sum (lane 0) at slot site/64
+__device__ __forceinline__ void put_partial(double* part, long vh, int r, long i, double v) {
+  v = wave_sum64(v);
+  if ((threadIdx.x & 63) == 0 && i < vh) part[(long)r * n_partials(vh) + i / 64] = v;
+}
+
+template <int P>
+__global__ void zero_kernel(Vec v, long vh) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cx<T> z[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) z[c] = {T(0), T(0)};
+  SpinorIO<P>::store(static_cast<char*>(v.p) + blockIdx.y * v.stride, vh, i, z);
+}
+
+template <int P>
+__global__ void norm2_kernel(Vec v, long vh, double* part) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  double s = 0.0;
+  if (i < vh) {
+    cx<T> a[12];
+    SpinorIO<P>::load(static_cast<const char*>(v.p) + r * v.stride, vh, i, a);
+    T t = T(0);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) t += norm2(a[c]);
+    s = (double)t;
+  }
+  put_partial(part, vh, r, i, s);
+}
+
+// one block of 256 threads per right-hand side; strided sequential sums then a
+// fixed LDS tree
+__device__ __forceinline__ double block_sum_partials(const double* part, long nw) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long w = threadIdx.x; w < nw; w += 256) s += part[w];
+  red[threadIdx.x] = s;
+  __syncthreads();
+#pragma unroll
+  for (int off = 128; off >= 1; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__global__ void sum_partials_kernel(const double* part, long nw, double* out) {
+  double s = block_sum_partials(part + (long)blockIdx.x * nw, nw);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+template <int P>
+__global__ void axpby_kernel(double a, Vec x, double b, Vec y, long vh) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  if (i >= vh) return;
+  cx<T> xv[12], yv[12];
+  SpinorIO<P>::load(static_cast<const char*>(x.p) + r * x.stride, vh, i, xv);
+  char* yp = static_cast<char*>(y.p) + r * y.stride;
+  SpinorIO<P>::load(yp, vh, i, yv);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) yv[c] = T(a) * xv[c] + T(b) * yv[c];
+  SpinorIO<P>::store(yp, vh, i, yv);
+}
+
+template <int P>
+__global__ void clover_apply_kernel(const void* clov, Vec in, Vec out, long vh) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  if (i >= vh) return;
+  T cl[72];
+  CloverIO<P>::load(clov, vh, i, cl);
+  cx<T> v[12], o[12];
+  SpinorIO<P>::load(static_cast<const char*>(in.p) + r * in.stride, vh, i, v);
+  clover_mul(cl, v, o);
+  SpinorIO<P>::store(static_cast<char*>(out.p) + r * out.stride, vh, i, o);
+}
+
+// ---------------------------------------------------------------- CGLS ------
+__global__ void cg_beta_kernel(const double* part, long nw, CgScal* sc) {
+  CgScal& s = sc[blockIdx.x];
+  if (!s.active) return;
+  double g = block_sum_partials(part + (long)blockIdx.x * nw, nw);
+  if (threadIdx.x == 0) {
+    if (s.first || s.converged || !(s.gamma > 0.0)) {
+      s.beta = 0.0;
+    } else {
+      s.beta = g / s.gamma;
+    }
+    s.first = 0;
+    s.gamma = g;
+  }
+}
+
+__global__ void cg_alpha_kernel(const double* part, long nw, CgScal* sc) {
+  CgScal& s = sc[blockIdx.x];
+  if (!s.active) return;
+  double tt = block_sum_partials(part + (long)blockIdx.x * nw, nw);
+  if (threadIdx.x == 0) {
+    s.tt = tt;
+    if (s.converged) {
+      s.alpha = 0.0;
+    } else if (!(tt > 0.0) || !(s.gamma == s.gamma) || tt != tt || tt > 1.0e300) {
+      s.alpha = 0.0;  // breakdown (p^dag Mhat^dag Mhat p <= 0 or not finite): freeze
+      s.breakdown = 1;
+    } else {
+      s.alpha = s.gamma / tt;
+    }
+  }
+}
+
+__global__ void cg_ss_kernel(const double* part, long nw, CgScal* sc) {
+  CgScal& s = sc[blockIdx.x];
+  if (!s.active) return;
+  double ss = block_sum_partials(part + (long)blockIdx.x * nw, nw);
+  if (threadIdx.x == 0) {
+    if (!s.converged) {
+      s.ss = ss;
+      s.iter += 1;
+      if (ss <= s.target2) s.converged = 1;
+    }
+  }
+}
+
+template <int P>
+__global__ void cg_xpay_kernel(Vec r, Vec p, const CgScal* sc, long vh) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (i >= vh || !sc[k].active) return;
+  const T beta = T(sc[k].beta);
+  cx<T> rv[12], pv[12];
+  SpinorIO<P>::load(static_cast<const char*>(r.p) + k * r.stride, vh, i, rv);
+  char* pp = static_cast<char*>(p.p) + k * p.stride;
+  SpinorIO<P>::load(pp, vh, i, pv);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) pv[c] = rv[c] + beta * pv[c];
+  SpinorIO<P>::store(pp, vh, i, pv);
+}
+
+template <int P, int PX>
+__global__ void cg_update_kernel(Vec x, Vec p, Vec t, Vec s, const CgScal* sc, double* part, long vh) {
+  using T = typename Real<P>::T;
+  using TX = typename Real<PX>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (!sc[k].active) return;
+  double acc = 0.0;
+  if (i < vh) {
+    const double alpha = sc[k].alpha;
+    cx<T> pv[12], tv[12], sv[12];
+    cx<TX> xv[12];
+    SpinorIO<P>::load(static_cast<const char*>(p.p) + k * p.stride, vh, i, pv);
+    SpinorIO<P>::load(static_cast<const char*>(t.p) + k * t.stride, vh, i, tv);
+    char* sp = static_cast<char*>(s.p) + k * s.stride;
+    char* xp = static_cast<char*>(x.p) + k * x.stride;
+    SpinorIO<P>::load(sp, vh, i, sv);
+    SpinorIO<PX>::load(xp, vh, i, xv);
+    const T a = T(alpha);
+    const TX ax = TX(alpha);
+    T n = T(0);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      xv[c] = {xv[c].re + ax * TX(pv[c].re), xv[c].im + ax * TX(pv[c].im)};
+      sv[c] = sv[c] - a * tv[c];
+      n += norm2(sv[c]);
+    }
+    SpinorIO<PX>::store(xp, vh, i, xv);
+    SpinorIO<P>::store(sp, vh, i, sv);
+    acc = (double)n;
+  }
+  put_partial(part, vh, k, i, acc);
+}
+
+// ------------------------------------------------------------- launchers ----
+#define QCU_PSW(PV, K, ...)                                                      \
+  do {                                                                           \
+    if ((PV) == F64) hipLaunchKernelGGL(K<F64>, __VA_ARGS__);                    \
+    else if ((PV) == F32) hipLaunchKernelGGL(K<F32>, __VA_ARGS__);               \
+    else hipLaunchKernelGGL(K<F16>, __VA_ARGS__);                                \
+  } while (0)
+
+void launch_zero(int prec, Vec v, long vh, int nrhs, hipStream_t st) {
+  QCU_PSW(prec, zero_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, v, vh);
+}
+void launch_norm2(int prec, Vec v, long vh, int nrhs, double* part, hipStream_t st) {
+  QCU_PSW(prec, norm2_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, v, vh, part);
+}
+void launch_sum_partials(const double* part, long nw, int nrhs, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, out);
+}
+void launch_axpby(int prec, double a, Vec x, double b, Vec y, long vh, int nrhs, hipStream_t st) {
+  QCU_PSW(prec, axpby_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, a, x, b, y, vh);
+}
+void launch_clover_apply(int prec, const void* clov, Vec in, Vec out, long vh, int nrhs, hipStream_t st) {
+  QCU_PSW(prec, clover_apply_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, clov, in, out, vh);
+}
+void launch_cg_beta(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, sc);
+}
+void launch_cg_alpha(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, sc);
+}
+void launch_cg_ss(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
+  hipLaunchKernelGGL(cg_ss_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, sc);
+}
+void launch_cg_xpay(int prec, Vec r, Vec p, const CgScal* sc, long vh, int nrhs, hipStream_t st) {
+  QCU_PSW(prec, cg_xpay_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, r, p, sc, vh);
+}
+void launch_cg_update(int prec, int xprec, Vec x, Vec p, Vec t, Vec s, const CgScal* sc, double* part, long vh,
+                      int nrhs, hipStream_t st) {
+  dim3 g(nblk(vh, 256), nrhs), b(256);
+  if (prec == F64) hipLaunchKernelGGL((cg_update_kernel<F64, F64>), g, b, 0, st, x, p, t, s, sc, part, vh);
+  else if (prec == F32 && xprec == F64) hipLaunchKernelGGL((cg_update_kernel<F32, F64>), g, b, 0, st, x, p, t, s, sc, part, vh);
+  else if (prec == F32) hipLaunchKernelGGL((cg_update_kernel<F32, F32>), g, b, 0, st, x, p, t, s, sc, part, vh);
+  else if (xprec == F64) hipLaunchKernelGGL((cg_update_kernel<F16, F64>), g, b, 0, st, x, p, t, s, sc, part, vh);
+  else hipLaunchKernelGGL((cg_update_kernel<F16, F32>), g, b, 0, st, x, p, t, s, sc, part, vh);
+}
+
+}  // namespace qcu

@@ -1,0 +1,8 @@
+// k_hop_f32.hip -- F32 instantiations of the hopping kernel (k_hop.inc).
+#include "k_hop.inc"
+
+namespace qcu {
+void launch_hop_f32(int recon, int epi, const HopArgs& a, int block, int rb, hipStream_t st) {
+  launch_hop_prec<F32>(recon, epi, a, block, rb, st);
+}
+}  // namespace qcu

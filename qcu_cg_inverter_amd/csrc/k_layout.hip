@@ -1,0 +1,164 @@
+// k_layout.hip -- conversions between the interface layouts (complex128 AoS, as the
+// reference's entry points take them: include/qcu_point.cuh:65-72) and the internal
+// SoA layouts of each precision, plus precision conversions between internal fields.
+// Role of src/qcu_shift_storage.cu:74-194 (AoS <-> SoA, gauge packing); written for
+// wave64 with one thread per site.
+#include "qcu_fields.h"
+#include "qcu_kernels.h"
+
+namespace qcu {
+
+static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <int P>
+__global__ void spinor_from_aos_kernel(const double2* aos, long aos_stride, Vec out, long vh) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  if (i >= vh) return;
+  const double2* s = reinterpret_cast<const double2*>(reinterpret_cast<const char*>(aos) + r * aos_stride) + i * 12;
+  cx<T> v[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    double2 q = s[c];
+    v[c] = {T(q.x), T(q.y)};
+  }
+  SpinorIO<P>::store(static_cast<char*>(out.p) + r * out.stride, vh, i, v);
+}
+
+template <int P>
+__global__ void spinor_to_aos_kernel(Vec in, double2* aos, long aos_stride, long vh) {
+  using T = typename Real<P>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  if (i >= vh) return;
+  cx<T> v[12];
+  SpinorIO<P>::load_cached(static_cast<const char*>(in.p) + r * in.stride, vh, i, v);
+  double2* d = reinterpret_cast<double2*>(reinterpret_cast<char*>(aos) + r * aos_stride) + i * 12;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) d[c] = double2{(double)v[c].re, (double)v[c].im};
+}
+
+void launch_spinor_from_aos(int prec, const void* aos, long aos_stride, Vec out, long vh, int nrhs, hipStream_t st) {
+  dim3 grid(nblk(vh, 256), nrhs);
+  const double2* a = static_cast<const double2*>(aos);
+  if (prec == F64) hipLaunchKernelGGL(spinor_from_aos_kernel<F64>, grid, dim3(256), 0, st, a, aos_stride, out, vh);
+  else if (prec == F32) hipLaunchKernelGGL(spinor_from_aos_kernel<F32>, grid, dim3(256), 0, st, a, aos_stride, out, vh);
+  else hipLaunchKernelGGL(spinor_from_aos_kernel<F16>, grid, dim3(256), 0, st, a, aos_stride, out, vh);
+}
+
+void launch_spinor_to_aos(int prec, Vec in, void* aos, long aos_stride, long vh, int nrhs, hipStream_t st) {
+  dim3 grid(nblk(vh, 256), nrhs);
+  double2* a = static_cast<double2*>(aos);
+  if (prec == F64) hipLaunchKernelGGL(spinor_to_aos_kernel<F64>, grid, dim3(256), 0, st, in, a, aos_stride, vh);
+  else if (prec == F32) hipLaunchKernelGGL(spinor_to_aos_kernel<F32>, grid, dim3(256), 0, st, in, a, aos_stride, vh);
+  else hipLaunchKernelGGL(spinor_to_aos_kernel<F16>, grid, dim3(256), 0, st, in, a, aos_stride, vh);
+}
+
+// gauge: thread per (link, mu, parity); the interface link is 9 complex row-major
+template <int P, int R>
+__global__ void gauge_pack_kernel(const double2* aos, void* out, long vh) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int mp = blockIdx.y;  // mu * 2 + parity
+  if (i >= vh) return;
+  const double2* s = aos + ((long)mp * vh + i) * 9;
+  constexpr int NE = R == 12 ? 6 : 9;
+  if (P == F64) {
+    double2* d = static_cast<double2*>(out) + (long)mp * NE * vh + i;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) d[e * vh] = s[e];
+  } else if (P == F32) {
+    if (R == 12) {
+      float4* d = static_cast<float4*>(out) + (long)mp * 3 * vh + i;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double2 a = s[2 * k], b = s[2 * k + 1];
+        d[k * vh] = float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
+      }
+    } else {
+      float2* d = static_cast<float2*>(out) + (long)mp * 9 * vh + i;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) d[e * vh] = float2{(float)s[e].x, (float)s[e].y};
+    }
+  } else {
+    if (R == 12) {
+      uint2* d = static_cast<uint2*>(out) + (long)mp * 3 * vh + i;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double2 a = s[2 * k], b = s[2 * k + 1];
+        unsigned lo = (unsigned)f2h((float)a.x) | ((unsigned)f2h((float)a.y) << 16);
+        unsigned hi = (unsigned)f2h((float)b.x) | ((unsigned)f2h((float)b.y) << 16);
+        d[k * vh] = uint2{lo, hi};
+      }
+    } else {
+      unsigned* d = static_cast<unsigned*>(out) + (long)mp * 9 * vh + i;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) d[e * vh] = (unsigned)f2h((float)s[e].x) | ((unsigned)f2h((float)s[e].y) << 16);
+    }
+  }
+}
+
+void launch_gauge_pack(int prec, int recon, const void* aos, void* out, long vh, hipStream_t st) {
+  dim3 grid(nblk(vh, 256), 8);
+  const double2* a = static_cast<const double2*>(aos);
+#define QCU_GP(P, R) \
+  if (prec == P && recon == R) { hipLaunchKernelGGL((gauge_pack_kernel<P, R>), grid, dim3(256), 0, st, a, out, vh); return; }
+  QCU_GP(F64, 18) QCU_GP(F64, 12) QCU_GP(F32, 18) QCU_GP(F32, 12) QCU_GP(F16, 18) QCU_GP(F16, 12)
+#undef QCU_GP
+}
+
+// row 2 of every interface-layout link rebuilt from rows 0 and 1, the reference's
+// reading of a gauge (src/qcu_wilson_dslash_neo.cu:73-86)
+__global__ void gauge_reconstruct_kernel(double2* aos, long nlinks) {
+  long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlinks) return;
+  double2* s = aos + l * 9;
+  cx<double> u[9];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) u[e] = {s[e].x, s[e].y};
+  recon_row2(u);
+#pragma unroll
+  for (int e = 6; e < 9; ++e) s[e] = double2{u[e].re, u[e].im};
+}
+
+void launch_gauge_reconstruct(void* aos, long vh, hipStream_t st) {
+  long n = 8 * vh;
+  hipLaunchKernelGGL(gauge_reconstruct_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, static_cast<double2*>(aos), n);
+}
+
+template <int PI, int PO>
+__global__ void spinor_convert_kernel(Vec in, Vec out, long vh, int accumulate) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  if (i >= vh) return;
+  using TI = typename Real<PI>::T;
+  using TO = typename Real<PO>::T;
+  cx<TI> a[12];
+  SpinorIO<PI>::load(static_cast<const char*>(in.p) + r * in.stride, vh, i, a);
+  cx<TO> b[12];
+  char* o = static_cast<char*>(out.p) + r * out.stride;
+  if (accumulate) {
+    SpinorIO<PO>::load(o, vh, i, b);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) b[c] = {b[c].re + TO(a[c].re), b[c].im + TO(a[c].im)};
+  } else {
+#pragma unroll
+    for (int c = 0; c < 12; ++c) b[c] = {TO(a[c].re), TO(a[c].im)};
+  }
+  SpinorIO<PO>::store(o, vh, i, b);
+}
+
+void launch_spinor_convert(int pin, Vec in, int pout, Vec out, long vh, int nrhs, int accumulate, hipStream_t st) {
+  dim3 grid(nblk(vh, 256), nrhs);
+#define QCU_SC(A, B)                                                                                       \
+  if (pin == A && pout == B) {                                                                             \
+    hipLaunchKernelGGL((spinor_convert_kernel<A, B>), grid, dim3(256), 0, st, in, out, vh, accumulate); \
+    return;                                                                                                \
+  }
+  QCU_SC(F64, F64) QCU_SC(F64, F32) QCU_SC(F64, F16)
+  QCU_SC(F32, F64) QCU_SC(F32, F32) QCU_SC(F32, F16)
+  QCU_SC(F16, F64) QCU_SC(F16, F32) QCU_SC(F16, F16)
+#undef QCU_SC
+}
+
+}  // namespace qcu

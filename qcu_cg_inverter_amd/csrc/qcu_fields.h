@@ -1,0 +1,340 @@
+// qcu_fields.h -- device-side complex arithmetic and the per-precision loads and
+// stores of spinor, link and clover sites (layouts in qcu_common.h).
+#pragma once
+
+#include "qcu_common.h"
+
+namespace qcu {
+
+template <typename T>
+struct cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(T s, cx<T> a) { return {s * a.re, s * a.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cx<T> mulc(cx<T> a, cx<T> b) {
+  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// multiply by i^k (k = 0..3)
+template <typename T>
+__device__ __forceinline__ cx<T> iph(cx<T> a, int k) {
+  k &= 3;
+  return k == 0 ? a : k == 1 ? cx<T>{-a.im, a.re} : k == 2 ? cx<T>{-a.re, -a.im} : cx<T>{a.im, -a.re};
+}
+template <typename T>
+__device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; }
+
+// streaming loads (nontemporal hint) of the HIP vector types through clang vectors
+typedef double v2d_t __attribute__((ext_vector_type(2)));
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+typedef unsigned v4u_t __attribute__((ext_vector_type(4)));
+typedef unsigned v2u_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+  v2d_t v = __builtin_nontemporal_load(reinterpret_cast<const v2d_t*>(p));
+  return double2{v.x, v.y};
+}
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+  v4f_t v = __builtin_nontemporal_load(reinterpret_cast<const v4f_t*>(p));
+  return float4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+  v2f_t v = __builtin_nontemporal_load(reinterpret_cast<const v2f_t*>(p));
+  return float2{v.x, v.y};
+}
+__device__ __forceinline__ uint4 ld_nt(const uint4* p) {
+  v4u_t v = __builtin_nontemporal_load(reinterpret_cast<const v4u_t*>(p));
+  return uint4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ uint2 ld_nt(const uint2* p) {
+  v2u_t v = __builtin_nontemporal_load(reinterpret_cast<const v2u_t*>(p));
+  return uint2{v.x, v.y};
+}
+__device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return __builtin_nontemporal_load(p); }
+
+template <int P>
+struct Real { using T = float; };
+template <>
+struct Real<F64> { using T = double; };
+
+// ---------------------------------------------------------------- spinors ---
+template <int P>
+struct SpinorIO;
+
+template <>
+struct SpinorIO<F64> {
+  using T = double;
+  static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
+    const double2* b = static_cast<const double2*>(base) + i;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      double2 q = ld_nt(b + c * vh);
+      v[c] = {q.x, q.y};
+    }
+  }
+  static __device__ __forceinline__ void load_cached(const void* base, long vh, long i, cx<T> v[12]) {
+    const double2* b = static_cast<const double2*>(base) + i;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      double2 q = b[c * vh];
+      v[c] = {q.x, q.y};
+    }
+  }
+  static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
+    double2* b = static_cast<double2*>(base) + i;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) b[c * vh] = double2{v[c].re, v[c].im};
+  }
+};
+
+template <>
+struct SpinorIO<F32> {
+  using T = float;
+  static __device__ __forceinline__ void load_cached(const void* base, long vh, long i, cx<T> v[12]) {
+    const float4* b = static_cast<const float4*>(base) + i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      float4 q = b[k * vh];
+      v[2 * k] = {q.x, q.y};
+      v[2 * k + 1] = {q.z, q.w};
+    }
+  }
+  static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
+    load_cached(base, vh, i, v);
+  }
+  static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
+    float4* b = static_cast<float4*>(base) + i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k * vh] = float4{v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im};
+  }
+};
+
+__device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
+__device__ __forceinline__ unsigned short f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
+__device__ __forceinline__ void unpack8(uint4 q, float f[8]) {
+  unsigned w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[2 * k] = h2f((unsigned short)(w[k] & 0xffff));
+    f[2 * k + 1] = h2f((unsigned short)(w[k] >> 16));
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+  unsigned w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = (unsigned)f2h(f[2 * k]) | ((unsigned)f2h(f[2 * k + 1]) << 16);
+  return uint4{w[0], w[1], w[2], w[3]};
+}
+
+// half precision with one float scale per site: value = half * norm,
+// norm = max |real component| of the site
+template <>
+struct SpinorIO<F16> {
+  using T = float;
+  static __device__ __forceinline__ const float* norms(const void* base, long vh) {
+    return reinterpret_cast<const float*>(static_cast<const uint4*>(base) + 3 * vh);
+  }
+  static __device__ __forceinline__ void load_cached(const void* base, long vh, long i, cx<T> v[12]) {
+    const uint4* b = static_cast<const uint4*>(base) + i;
+    float nrm = norms(base, vh)[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float f[8];
+      unpack8(b[k * vh], f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[4 * k + j] = {f[2 * j] * nrm, f[2 * j + 1] * nrm};
+    }
+  }
+  static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
+    load_cached(base, vh, i, v);
+  }
+  static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
+    float m = 0.f;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
+    float inv = m > 0.f ? 1.0f / m : 0.f;
+    uint4* b = static_cast<uint4*>(base) + i;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float f[8];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        f[2 * j] = v[4 * k + j].re * inv;
+        f[2 * j + 1] = v[4 * k + j].im * inv;
+      }
+      b[k * vh] = pack8(f);
+    }
+    reinterpret_cast<float*>(static_cast<uint4*>(base) + 3 * vh)[i] = m;
+  }
+};
+
+// ------------------------------------------------------------------ links ---
+// row 2 of an SU(3) link from rows 0 and 1: conj(row0 x row1)
+template <typename T>
+__device__ __forceinline__ void recon_row2(cx<T> u[9]) {
+  u[6] = conj(u[1] * u[5] - u[2] * u[4]);
+  u[7] = conj(u[2] * u[3] - u[0] * u[5]);
+  u[8] = conj(u[0] * u[4] - u[1] * u[3]);
+}
+
+template <int P, int R>
+struct LinkIO;
+
+template <int R>
+struct LinkIO<F64, R> {
+  using T = double;
+  static constexpr int NE = R == 12 ? 6 : 9;
+  // plane base of (mu, parity)
+  static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
+    const double2* b = static_cast<const double2*>(g) + (long)(mu * 2 + par) * NE * vh + i;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      double2 q = ld_nt(b + e * vh);
+      u[e] = {q.x, q.y};
+    }
+    if (R == 12) recon_row2(u);
+  }
+};
+
+template <int R>
+struct LinkIO<F32, R> {
+  using T = float;
+  static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
+    if (R == 12) {
+      const float4* b = static_cast<const float4*>(g) + (long)(mu * 2 + par) * 3 * vh + i;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float4 q = ld_nt(b + k * vh);
+        u[2 * k] = {q.x, q.y};
+        u[2 * k + 1] = {q.z, q.w};
+      }
+      recon_row2(u);
+    } else {
+      const float2* b = static_cast<const float2*>(g) + (long)(mu * 2 + par) * 9 * vh + i;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        float2 q = ld_nt(b + e * vh);
+        u[e] = {q.x, q.y};
+      }
+    }
+  }
+};
+
+template <int R>
+struct LinkIO<F16, R> {
+  using T = float;
+  static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
+    if (R == 12) {
+      const uint2* b = static_cast<const uint2*>(g) + (long)(mu * 2 + par) * 3 * vh + i;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        uint2 q = ld_nt(b + k * vh);
+        u[2 * k] = {h2f(q.x & 0xffff), h2f(q.x >> 16)};
+        u[2 * k + 1] = {h2f(q.y & 0xffff), h2f(q.y >> 16)};
+      }
+      recon_row2(u);
+    } else {
+      const unsigned* b = static_cast<const unsigned*>(g) + (long)(mu * 2 + par) * 9 * vh + i;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        unsigned q = ld_nt(b + e * vh);
+        u[e] = {h2f(q & 0xffff), h2f(q >> 16)};
+      }
+    }
+  }
+};
+
+// ----------------------------------------------------------------- clover ---
+// 72 reals per site: block b at reals [36b, 36b+36): d0..d5, then the 15 strictly
+// lower entries (1,0),(2,0),(2,1),(3,0),...,(5,4) as (re, im) pairs.
+template <int P>
+struct CloverIO;
+
+template <>
+struct CloverIO<F64> {
+  using T = double;
+  static __device__ __forceinline__ void load(const void* c, long vh, long i, T a[72]) {
+    const double2* b = static_cast<const double2*>(c) + i;
+#pragma unroll
+    for (int k = 0; k < 36; ++k) {
+      double2 q = ld_nt(b + k * vh);
+      a[2 * k] = q.x;
+      a[2 * k + 1] = q.y;
+    }
+  }
+};
+template <>
+struct CloverIO<F32> {
+  using T = float;
+  static __device__ __forceinline__ void load(const void* c, long vh, long i, T a[72]) {
+    const float4* b = static_cast<const float4*>(c) + i;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+      float4 q = ld_nt(b + k * vh);
+      a[4 * k] = q.x;
+      a[4 * k + 1] = q.y;
+      a[4 * k + 2] = q.z;
+      a[4 * k + 3] = q.w;
+    }
+  }
+};
+template <>
+struct CloverIO<F16> {
+  using T = float;
+  static __device__ __forceinline__ void load(const void* c, long vh, long i, T a[72]) {
+    const uint4* b = static_cast<const uint4*>(c) + i;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) unpack8(ld_nt(b + k * vh), a + 8 * k);
+  }
+};
+
+// out = blockdiag(M0, M1) v for one site, M packed as above
+template <typename T>
+__device__ __forceinline__ void clover_mul(const T a[72], const cx<T> v[12], cx<T> out[12]) {
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const T* m = a + 36 * b;
+    const cx<T>* x = v + 6 * b;
+    cx<T> y[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) y[r] = m[r] * x[r];
+    int k = 6;
+#pragma unroll
+    for (int r = 1; r < 6; ++r) {
+#pragma unroll
+      for (int c = 0; c < r; ++c) {
+        cx<T> l = {m[k], m[k + 1]};
+        k += 2;
+        y[r] = y[r] + l * x[c];
+        y[c] = y[c] + conj(l) * x[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) out[6 * b + r] = y[r];
+  }
+}
+
+// XCD-aware block order: the hardware hands workgroup b to XCD (b mod 8); remap so
+// that each XCD walks one contiguous range of sites (its L2 then sees neighbours).
+__device__ __forceinline__ unsigned remap_block(unsigned b, unsigned nb, int on) {
+  if (!on) return b;
+  unsigned x = b & 7u, k = b >> 3, q = nb >> 3, r = nb & 7u;
+  return x * q + (x < r ? x : r) + k;
+}
+
+}  // namespace qcu

@@ -1,0 +1,109 @@
+// qcu_host.h -- host-side runtime of libqcu.so: context, resident fields, errors.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qcu_ex.h"
+#include "qcu_kernels.h"
+
+namespace qcu {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char* what, const char* file, int line);
+#define QHIP(x) ::qcu::check_hip((x), #x, __FILE__, __LINE__)
+#define QREQ(cond, msg)                                                         \
+  do {                                                                          \
+    if (!(cond)) throw ::qcu::Error(QCU_ERROR_INVALID_VALUE, std::string(msg)); \
+  } while (0)
+
+// device allocation owned by the library (grow-only when reused as scratch)
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void alloc(size_t bytes);
+  void ensure(size_t bytes);
+  void release();
+  char* c() const { return static_cast<char*>(p); }
+};
+
+// A = 1 + ck T and A^-1 of both parities, packed, per precision (F64 built first,
+// the others converted on demand)
+struct CloverField {
+  double ck = 0.0;
+  int nbad = 0;
+  long vh = 0;
+  DevBuf A[3], Ainv[3];
+  const void* a(int prec, int parity) const { return A[prec].c() + parity * clover_bytes(prec, vh) / 2; }
+  const void* ainv(int prec, int parity) const { return Ainv[prec].c() + parity * clover_bytes(prec, vh) / 2; }
+};
+
+// a gauge field resident in HBM: the interface copy plus packed variants made on
+// first use, and clover fields keyed by kappa * c_sw
+struct GaugeField {
+  int L[4];
+  Geom geo;
+  DevBuf aos;
+  DevBuf packed[3][2];  // [precision][recon == 12]
+  std::map<double, std::unique_ptr<CloverField>> clovers;
+  const void* get(int prec, int recon);
+  CloverField* clover(double ck, int prec);
+  size_t bytes() const;
+};
+
+struct Tune {
+  int block = 256;
+  int rb = 1;
+  int xcd = 1;
+};
+
+struct Context {
+  bool inited = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  // state of the drop-in entry points
+  bool have_geom = false;
+  int L[4] = {0, 0, 0, 0};
+  std::map<const void*, std::unique_ptr<GaugeField>> dropin;
+  qcuSolveParams default_solve;
+  // scratch shared by calls (grow-only)
+  std::map<std::string, std::unique_ptr<DevBuf>> scratch;
+  std::map<std::string, Tune> tune;
+  DevBuf* buf(const std::string& name, size_t bytes);
+};
+
+Context& ctx();
+void ensure_init();
+inline hipStream_t stream() { return ctx().stream; }
+
+void validate_lattice(const int L[4]);
+std::unique_ptr<GaugeField> make_gauge_from_aos(const void* aos_device, const int L[4]);
+std::unique_ptr<GaugeField> make_gauge_random(const int L[4], uint64_t seed, double eps);
+
+Tune tuned_hop(int prec, int recon, long vh, int nrhs);
+void set_tuned_hop(int prec, int recon, long vh, int nrhs, const Tune& t);
+
+// one hop launch on nrhs strided fields
+void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
+         const void* clov, double coef, int nrhs, double* partials, hipStream_t st);
+
+// solver entry (solver.cpp)
+void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& sp, int nrhs, void* const* x,
+                 const void* const* b, qcuSolveStats* stats);
+void apply_mhat_iface(GaugeField& g, const qcuOperator& op, void* out, const void* in, int dagger, int prec);
+void apply_m_iface(GaugeField& g, const qcuOperator& op, void* out, const void* in, int dagger);
+
+}  // namespace qcu

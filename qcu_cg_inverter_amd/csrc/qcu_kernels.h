@@ -1,0 +1,77 @@
+// qcu_kernels.h -- host-side launchers of the HIP kernels (all asynchronous on the
+// given stream).  Fields are one half lattice in the internal layout unless noted;
+// a batch of right-hand sides is strided by `stride` bytes.
+#pragma once
+
+#include "qcu_common.h"
+
+namespace qcu {
+
+struct Vec {
+  void* p;
+  long stride;  // bytes between right-hand sides
+};
+inline Vec vec(void* p, long stride = 0) { return Vec{p, stride}; }
+
+// number of 64-site reduction partials of one half lattice
+__host__ __device__ inline long n_partials(long vh) { return (vh + 63) / 64; }
+
+// ------------------------------------------------------------------ hop -----
+void launch_hop(int prec, int recon, int epi, const HopArgs& a, int block, int rb, hipStream_t st);
+
+// --------------------------------------------------------------- layout -----
+// interface spinor (complex128 [Vh][4][3]) <-> internal layout of `prec`
+void launch_spinor_from_aos(int prec, const void* aos, long aos_stride, Vec out, long vh, int nrhs, hipStream_t st);
+void launch_spinor_to_aos(int prec, Vec in, void* aos, long aos_stride, long vh, int nrhs, hipStream_t st);
+// interface gauge (complex128 [4][2][Vh][3][3]) -> packed planes
+void launch_gauge_pack(int prec, int recon, const void* aos, void* out, long vh, hipStream_t st);
+// rebuild row 2 of every interface-layout link from rows 0 and 1 (in place)
+void launch_gauge_reconstruct(void* aos, long vh, hipStream_t st);
+// out = in (accumulate = 0) or out += in (accumulate = 1), any precisions
+void launch_spinor_convert(int pin, Vec in, int pout, Vec out, long vh, int nrhs, int accumulate, hipStream_t st);
+
+// ----------------------------------------------------------------- blas -----
+// per-right-hand-side CG scalars kept in device memory
+struct CgScal {
+  double alpha, beta, gamma, tt, ss, target2, ss_ref;
+  int first, converged, active, breakdown, iter, pad;
+};
+
+void launch_zero(int prec, Vec v, long vh, int nrhs, hipStream_t st);
+// partials[r][w] = sum over wave w of |v_r|^2
+void launch_norm2(int prec, Vec v, long vh, int nrhs, double* partials, hipStream_t st);
+// out[r] = sum_w partials[r][w] in a fixed order (one block per right-hand side)
+void launch_sum_partials(const double* partials, long nw, int nrhs, double* out, hipStream_t st);
+// y = a x + b y (F64 or same-precision fields)
+void launch_axpby(int prec, double a, Vec x, double b, Vec y, long vh, int nrhs, hipStream_t st);
+// out = C in with C packed clover of one parity (same precision as the fields)
+void launch_clover_apply(int prec, const void* clov, Vec in, Vec out, long vh, int nrhs, hipStream_t st);
+
+// CGLS steps (see solver.cpp).  All skip right-hand sides with active == 0.
+void launch_cg_beta(const double* part_rr, long nw, CgScal* sc, int nrhs, hipStream_t st);
+void launch_cg_alpha(const double* part_tt, long nw, CgScal* sc, int nrhs, hipStream_t st);
+void launch_cg_ss(const double* part_ss, long nw, CgScal* sc, int nrhs, hipStream_t st);
+// p = r + beta p
+void launch_cg_xpay(int prec, Vec r, Vec p, const CgScal* sc, long vh, int nrhs, hipStream_t st);
+// x += alpha p ; s -= alpha t ; partials of |s|^2
+void launch_cg_update(int prec, int xprec, Vec x, Vec p, Vec t, Vec s, const CgScal* sc, double* part_ss, long vh,
+                      int nrhs, hipStream_t st);
+
+// --------------------------------------------------------------- clover -----
+// G[pl] = Q - Q^dag for the 6 planes at every site of `parity`: double2 [6][9][Vh]
+void launch_clover_leaves(const void* gauge_f64_r18, void* G, const Geom& g, int parity, hipStream_t st);
+// A = 1 + ck T and A^-1 for one parity, packed F64; nbad counts blocks that needed
+// the pivoted fallback instead of Cholesky
+void launch_clover_assemble(const void* G, void* A, void* Ainv, long vh, double ck, int* nbad, hipStream_t st);
+// packed F64 -> packed F32 / F16 (one parity)
+void launch_clover_convert(int prec, const void* src, void* dst, long vh, hipStream_t st);
+// packed F64 -> reference layout complex128 [Vh][2][6][6] (one parity)
+void launch_clover_unpack(const void* packed, void* out, long vh, hipStream_t st);
+
+// ------------------------------------------------------------ generators ---
+// U_mu(x) = SU3-project(exp(i eps H)) into the interface gauge layout
+void launch_gauge_random(void* aos, long vh, uint64_t seed, double eps, hipStream_t st);
+// N(0,1) per real of n complex128 values, counter = element index
+void launch_source_gauss(void* aos, long n_complex, uint64_t seed, hipStream_t st);
+
+}  // namespace qcu

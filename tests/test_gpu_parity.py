@@ -1,0 +1,255 @@
+"""GPU parity: libqcu.so (through its C ABI) against the FP64 oracle on the same
+seeded inputs.  FP64 tolerance 1e-14 relative (norm-wise) for the hopping term and
+operators; FP32 1e-6, FP16 (+ per-site norm) 3e-3."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-14
+
+
+@pytest.fixture(scope="module")
+def q():
+    import qcu_cg_inverter_amd as q
+
+    q.init(0)
+    return q
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (8, 8, 8, 8), (6, 4, 2, 8), (16, 16, 16, 32)])
+def test_generators_match_oracle(q, L):
+    g = q.Gauge.random(L, seed=7, eps=0.2)
+    ref = orc.gauge_random(L, 7, 0.2)
+    assert np.abs(g.to_numpy() - ref).max() < 1e-14
+    v = q.source_gaussian(L, seed=3)
+    assert np.abs(v - orc.source_gauss(L, 3)).max() < 1e-13
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (8, 8, 8, 8), (6, 4, 2, 8), (2, 2, 2, 2)])
+@pytest.mark.parametrize("recon", [12, 18])
+def test_hop_f64_matches_oracle(q, L, recon):
+    g = q.Gauge.random(L, seed=11, eps=0.5)
+    gref = g.to_numpy()
+    v = orc.source_gauss(L, 5)
+    for parity in (0, 1):
+        for dag in (0, 1):
+            for tb in (1, -1):
+                out = q.hop(g, v[1 - parity], parity, dagger=dag, t_boundary=tb, recon=recon)
+                ref = orc.hop(v[1 - parity], gref, L, parity, s=-1 if dag else 1, tbc=float(tb), recon=recon)
+                assert rel(out, ref) < FP64_TOL, (parity, dag, tb)
+
+
+def test_hop_f64_16x32(q):
+    L = (16, 16, 16, 32)
+    g = q.Gauge.random(L, seed=1, eps=0.2)
+    gref = g.to_numpy()
+    v = orc.source_gauss(L, 2)
+    out = q.hop(g, v[1], 0, t_boundary=-1, recon=18)
+    ref = orc.hop(v[1], gref, L, 0, s=1, tbc=-1.0, recon=18)
+    assert rel(out, ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-6), (2, 3e-3)])
+@pytest.mark.parametrize("recon", [12, 18])
+def test_hop_low_precision(q, prec, tol, recon):
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=4, eps=0.3)
+    gref = g.to_numpy()
+    v = orc.source_gauss(L, 6)
+    for parity in (0, 1):
+        out = q.hop(g, v[1 - parity], parity, dagger=1, t_boundary=-1, recon=recon, precision=prec)
+        ref = orc.hop(v[1 - parity], gref, L, parity, s=-1, tbc=-1.0, recon=recon)
+        assert rel(out, ref) < tol
+
+
+@pytest.mark.parametrize("ck", [0.122, 1.0])
+def test_clover_matches_oracle(q, ck):
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=2, eps=0.25)
+    gref = g.to_numpy()
+    A, Ainv, nf = g.clover(ck, 1.0)
+    T = orc.clover_T(gref, L, recon=18)
+    Aref = orc.clover_A(T, ck)
+    assert np.abs(A - Aref).max() < 1e-14
+    inv_ref, nbad = orc.clover_invert(Aref, chol=True)
+    assert nf == nbad
+    cond = np.linalg.cond(Aref.reshape(-1, 6, 6)).max()
+    assert np.abs(Ainv - inv_ref).max() < 1e-14 * cond * max(1.0, np.abs(inv_ref).max())
+    assert np.abs(np.einsum("...ij,...jk->...ik", A, Ainv) - np.eye(6)).max() < 1e-12
+
+
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+def test_full_operator_matches_oracle(q, csw):
+    L = (8, 4, 4, 8)
+    kappa = 0.124
+    g = q.Gauge.random(L, seed=9, eps=0.25)
+    gref = g.to_numpy()
+    A = orc.clover_fields(gref, L, kappa, csw, recon=18)[0] if csw else None
+    v = orc.source_gauss(L, 1)
+    for dag in (0, 1):
+        for tb in (1, -1):
+            op = q.Operator(kappa, csw, tb, 18)
+            out = q.apply_m(g, op, v, dagger=dag)
+            ref = orc.full_op(v, gref, L, kappa, A=A, s=-1 if dag else 1, tbc=float(tb), recon=18)
+            assert rel(out, ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+@pytest.mark.parametrize("prec,tol", [(0, FP64_TOL), (1, 1e-6), (2, 3e-3)])
+def test_schur_operator_matches_oracle(q, csw, prec, tol):
+    L = (8, 8, 4, 4)
+    kappa = 0.122
+    g = q.Gauge.random(L, seed=12, eps=0.25)
+    gref = g.to_numpy()
+    A = Ainv = None
+    if csw:
+        A, Ainv = orc.clover_fields(gref, L, kappa, csw, recon=12)
+    v = orc.source_gauss(L, 8)[0]
+    for dag in (0, 1):
+        op = q.Operator(kappa, csw, -1, 12)
+        out = q.apply_mhat(g, op, v, dagger=dag, precision=prec)
+        ref = orc.schur_op(v, gref, L, kappa, Aee=None if A is None else A[0],
+                           Ainv_oo=None if Ainv is None else Ainv[1], s=-1 if dag else 1, tbc=-1.0, recon=12)
+        assert rel(out, ref) < (tol if prec else 10 * tol)
+
+
+# ---------------------------------------------------------------- drop-in ---
+def dev(a):
+    import qcu_cg_inverter_amd as q
+
+    return q.DeviceBuffer.from_numpy(np.ascontiguousarray(a, np.complex128))
+
+
+def test_dropin_dslash_and_full(q):
+    L = (8, 8, 8, 16)
+    gref = orc.gauge_random(L, 21, 0.25)
+    prm = q.param(L)
+    gd = dev(gref)
+    v = orc.source_gauss(L, 4)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    q.lib().loadQcuGauge(gd.ptr, ctypes.byref(prm))
+    vh = orc.half_volume(L)
+    for parity in (0, 1):
+        inp = dev(v[1 - parity])
+        out = q.DeviceBuffer(vh * 192)
+        q.lib().dslashQcu(out.ptr, inp.ptr, gd.ptr, ctypes.byref(prm), parity)
+        res = out.to_numpy((vh, 4, 3))
+        ref = orc.hop(v[1 - parity], gref, L, parity, s=1, tbc=1.0, recon=12)
+        assert rel(res, ref) < FP64_TOL
+    # full operator (A - D_s) with A = 1 + T, kappa = c_sw = 1
+    A = orc.clover_fields(gref, L, 1.0, 1.0, recon=12)[0]
+    inp = dev(v)
+    for dag in (0, 1):
+        out = q.DeviceBuffer(2 * vh * 192)
+        q.lib().fullDslashQcu(out.ptr, inp.ptr, gd.ptr, ctypes.byref(prm), dag)
+        res = out.to_numpy((2, vh, 4, 3))
+        ref = orc.full_op(v, gref, L, 1.0, A=A, s=-1 if dag else 1, tbc=1.0, recon=12)
+        assert rel(res, ref) < FP64_TOL
+
+
+def test_dropin_cg_inverter(q):
+    L = (4, 4, 4, 8)
+    gref = orc.gauge_random(L, 5, 0.25)
+    prm = q.param(L)
+    gd = dev(gref)
+    b = orc.source_gauss(L, 9)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    bd = dev(b)
+    vh = orc.half_volume(L)
+    xd = q.DeviceBuffer(2 * vh * 192)
+    q.lib().cg_inverter(bd.ptr, xd.ptr, gd.ptr, ctypes.byref(prm))
+    x = xd.to_numpy((2, vh, 4, 3))
+    assert np.array_equal(bd.to_numpy((2, vh, 4, 3)), b)  # b untouched
+    A, Ainv = orc.clover_fields(gref, L, 1.0, 1.0, recon=12)
+    r = b - orc.full_op(x, gref, L, 1.0, A=A, s=1, tbc=1.0, recon=12)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-11
+    xr, st = orc.cg_eo(b, gref, L, 1.0, A=A, Ainv=Ainv, tbc=1.0, recon=12, tol=1e-12, max_iter=50000)
+    assert rel(x, xr) < 1e-8
+
+
+# ----------------------------------------------------------------- solver ---
+def test_solve_config1_wilson_fp64(q):
+    """8^4 Wilson, kappa 0.12, eps 0.1, antiperiodic t, 1 Gaussian source, 1e-10."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=1, eps=0.1)
+    gref = g.to_numpy()
+    b = orc.source_gauss(L, 2)
+    op = q.Operator(0.12, 0.0, -1, 18)
+    sp = q.default_solve_params(tol=1e-10, precision=q.PREC_DOUBLE)
+    x, st = q.solve(g, op, b, sp)
+    assert st.converged and st.true_residual <= 1e-10
+    r = b - orc.full_op(x, gref, L, 0.12, s=1, tbc=-1.0, recon=18)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-10
+    xr, sr = orc.cg_eo(b, gref, L, 0.12, tbc=-1.0, recon=18, tol=1e-10)
+    assert abs(st.iterations - sr.iterations) <= 2
+    assert rel(x, xr) < 1e-8
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+def test_solve_mixed_precision(q, prec, csw):
+    L = (8, 8, 8, 8)
+    kappa = 0.122 if csw else 0.124
+    g = q.Gauge.random(L, seed=3, eps=0.25)
+    gref = g.to_numpy()
+    b = orc.source_gauss(L, 4)
+    op = q.Operator(kappa, csw, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=prec)
+    x, st = q.solve(g, op, b, sp)
+    assert st.converged and st.true_residual <= 1e-10 and st.reliable_updates >= 1
+    A = orc.clover_fields(gref, L, kappa, csw, recon=12)[0] if csw else None
+    r = b - orc.full_op(x, gref, L, kappa, A=A, s=1, tbc=-1.0, recon=12)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_equals_alone(q, prec):
+    """Each right-hand side gives the same bits solved alone or in a batch."""
+    L = (8, 8, 4, 8)
+    g = q.Gauge.random(L, seed=8, eps=0.25)
+    op = q.Operator(0.122, 1.0, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=prec)
+    bs = [orc.source_gauss(L, 100 + i) for i in range(3)]
+    xs, sts = q.solve_batch(g, op, bs, sp)
+    for i in range(3):
+        xa, sa = q.solve(g, op, bs[i], sp)
+        assert np.array_equal(xa, xs[i]), i
+        assert sa.iterations == sts[i].iterations
+        assert sts[i].true_residual <= 1e-10
+
+
+def test_graph_equals_eager(q):
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=6, eps=0.2)
+    op = q.Operator(0.124, 0.0, -1, 18)
+    b = orc.source_gauss(L, 1)
+    xg, sg = q.solve(g, op, b, q.default_solve_params(tol=1e-10, precision=1, graph=1))
+    xe, se = q.solve(g, op, b, q.default_solve_params(tol=1e-10, precision=1, graph=0))
+    assert np.array_equal(xg, xe) and sg.iterations == se.iterations
+
+
+def test_error_paths(q):
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=6, eps=0.2)
+    op = q.Operator(0.124, 0.0, -1, 18)
+    b = orc.source_gauss(L, 1)
+    b[0, 3, 1, 2] = np.nan
+    with pytest.raises(q.QcuError) as e:
+        q.solve(g, op, b)
+    assert e.value.code == 6
+    with pytest.raises(q.QcuError) as e:
+        q.Gauge.random((5, 4, 4, 4), seed=1, eps=0.1)
+    assert e.value.code == 1
+    b = orc.source_gauss(L, 1)
+    x, st = q.solve(g, op, b, q.default_solve_params(tol=1e-12, max_iter=3), raise_on_error=False)
+    assert not st.converged and st.iterations <= 8
+    assert q.lib().qcuGetLastError() == 4
