@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the qcu hot path on one MI355X.
+
+Workload (step): one even-odd hopping-term application D_eo / D_oe (FP64, 18-real
+links, antiperiodic t) on the 16^3x32 lattice of the Wilson CG configuration
+(seeded near-unit SU(3) gauge, eps = 0.2).  `value` is the algorithmic Dslash
+bandwidth: 2880 B per output half-site (8 links x 144 B + 8 neighbour spinors x
+192 B + 192 B written) x Vh / time per application.
+
+The same JSON line also carries:
+  roofline      the hop kernel against the 8 TB/s HBM3E peak, HIP-event timed
+  cpu_baseline  the FP64 oracle hop timed on the host cores (bounded sample)
+  cg            FP64 vs mixed-precision (FP32 inner, FP64 reliable updates) CG to a
+                true residual of 1e-10 on the same lattice, kappa = 0.124, one
+                Gaussian source
+  dslash_large  the same kernel on 24^3x48 and 32^3x64 (working set above the
+                256 MB Infinity Cache), FP64 / FP32 / FP16 and reconstruct-12
+  copy          device-to-device copy bandwidth for scale
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Dslash GB/s & %HBM peak; CG time-to-1e-10 mixed vs FP64; p50/p99 solve ms"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_SITE = {("f64", 18): 2880, ("f64", 12): 2496, ("f32", 18): 1440, ("f32", 12): 1248,
+                  ("f16", 18): 756, ("f16", 12): 660}
+PREC = {"f64": 0, "f32": 1, "f16": 2}
+L_MAIN = (16, 16, 16, 32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = platform.processor()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def measured_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 PMC counters (committed under
+    profiles/), or None when no measurement for this kernel/workload exists."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds_target=15.0):
+    """Oracle FP64 hop on the host cores, on a bounded sample of the workload:
+    repeated applications on the 16^3x32 lattice for about seconds_target."""
+    import numpy as np
+
+    import oracle as orc
+
+    threads = min(os.cpu_count() or 1, 16)
+    orc.set_threads(threads)
+    L = L_MAIN
+    g = orc.gauge_random(L, 1, 0.2)
+    v = orc.source_gauss(L, 2)
+    orc.hop(v[1], g, L, 0, 1, -1.0, 18)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.hop(v[1 - (n & 1)], g, L, n & 1, 1, -1.0, 18)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target:
+            break
+    vh = orc.half_volume(L)
+    gbs = BYTES_PER_SITE[("f64", 18)] * vh * n / el / 1e9
+    del np
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 hop applications on 16^3x32 ({el:.1f} s, oracle, OpenMP)",
+            "ms_per_apply": round(1e3 * el / n, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--quick", action="store_true", help="skip the large lattices and the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiler driver: 1 GiB copy kernel x5 then the 16^3x32 FP64 hop x steps, nothing else")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build keeps the lattice on one GPU; --gpus {args.gpus} is not supported "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import qcu_cg_inverter_amd as q
+
+    q.init(0)
+    if args.kernels_only:
+        q.bench_copy(1 << 30, iters=5)
+        gk = q.Gauge.random(L_MAIN, seed=1, eps=0.2)
+        ms = q.bench_hop(gk, precision=0, recon=18, nrhs=1, iters=args.steps)
+        log(f"kernels-only: hop f64 r18 16^3x32 {ms*1e3:.2f} us/launch over {args.steps}")
+        return 0
+    out = {"metric": METRIC, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded near-unit SU(3) gauge eps=0.2, Gaussian spinors)"}
+    vh = q.half_volume(L_MAIN)
+    g = q.Gauge.random(L_MAIN, seed=1, eps=0.2)
+
+    # ---- headline: FP64 recon-18 hop on 16^3x32 --------------------------------
+    q.bench_hop(g, precision=0, recon=18, nrhs=1, iters=max(1, args.warmup))
+    q.check(q.lib().qcuSynchronize())
+    t0 = time.perf_counter()
+    ms_event = q.bench_hop(g, precision=0, recon=18, nrhs=1, iters=args.steps)
+    q.check(q.lib().qcuSynchronize())
+    wall = time.perf_counter() - t0
+    bps = BYTES_PER_SITE[("f64", 18)] * vh
+    achieved = bps / (ms_event * 1e-3) / 1e9
+    out["value"] = round(bps / (wall / args.steps) / 1e9, 2)
+    out["ms_per_step"] = round(1e3 * wall / args.steps, 5)
+    out["config"] = {"workload": "even-odd Wilson hop D_eo/D_oe, FP64, 18-real links, 16^3x32, antiperiodic t",
+                     "lattice": list(L_MAIN), "half_volume": vh, "bytes_per_site": 2880,
+                     "flop_per_site": 1320}
+    traffic = measured_traffic("hop_f64_r18_16x32")
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                       "kernel": "hop_kernel<F64,18,EPI_PLAIN>", "avg_launch_us": round(ms_event * 1e3, 3),
+                       "algorithmic_bytes_per_launch": bps}
+    log(f"hop f64 r18 16^3x32: {ms_event*1e3:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
+
+    # ---- copy bandwidth -----------------------------------------------------------
+    nb = 1 << 30
+    ms = q.bench_copy(nb, iters=20)
+    out["copy"] = {"bytes": nb, "GBps_read_plus_write": round(2 * nb / (ms * 1e-3) / 1e9, 1)}
+
+    # ---- CG: FP64 vs mixed on 16^3x32 Wilson ----------------------------------------
+    b = q.source_gaussian(L_MAIN, seed=2)
+    op = q.Operator(0.124, 0.0, -1, 12)
+    cg = {}
+    for name, prec in (("fp64", 0), ("mixed_fp32", 1), ("mixed_fp16", 2)):
+        sp = q.default_solve_params(tol=1e-10, precision=prec)
+        if prec == 0:
+            op64 = q.Operator(0.124, 0.0, -1, 18)
+            q.solve(g, op64, b, sp)  # warm (graph capture paths, buffers)
+            x, st = q.solve(g, op64, b, sp)
+        else:
+            q.solve(g, op, b, sp)
+            x, st = q.solve(g, op, b, sp)
+        cg[name] = {"seconds": round(st.seconds, 5), "iterations": st.iterations,
+                    "reliable_updates": st.reliable_updates, "true_residual": st.true_residual,
+                    "converged": bool(st.converged)}
+        log(f"cg {name}: {st.seconds*1e3:.2f} ms, {st.iterations} it, true res {st.true_residual:.2e}")
+    cg["speedup_mixed_fp32_vs_fp64"] = round(cg["fp64"]["seconds"] / cg["mixed_fp32"]["seconds"], 3)
+    cg["speedup_mixed_fp16_vs_fp64"] = round(cg["fp64"]["seconds"] / cg["mixed_fp16"]["seconds"], 3)
+    cg["problem"] = "16^3x32 Wilson, kappa=0.124, eps=0.2, antiperiodic t, 1 Gaussian source, tol 1e-10"
+    out["cg"] = cg
+    del np
+
+    # ---- larger lattices (beyond the Infinity Cache) ---------------------------------
+    if not args.quick:
+        large = {}
+        for L in ((24, 24, 24, 48), (32, 32, 32, 64)):
+            gl = q.Gauge.random(L, seed=3, eps=0.25)
+            vhl = q.half_volume(L)
+            key = "x".join(map(str, L))
+            large[key] = {}
+            for prec in ("f64", "f32", "f16"):
+                for recon in (18, 12):
+                    ms = q.bench_hop(gl, precision=PREC[prec], recon=recon, nrhs=1, iters=100)
+                    gbs = BYTES_PER_SITE[(prec, recon)] * vhl / (ms * 1e-3) / 1e9
+                    large[key][f"{prec}_r{recon}"] = {"us": round(ms * 1e3, 2), "GBps": round(gbs, 1),
+                                                       "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1)}
+                    log(f"hop {prec} r{recon} {key}: {ms*1e3:.1f} us, {gbs:.0f} GB/s")
+            gl.destroy()
+        out["dslash_large"] = large
+
+    if not (args.quick or args.no_cpu):
+        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"]["host"] = cpu_info()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

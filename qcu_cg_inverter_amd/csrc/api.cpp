@@ -611,14 +611,14 @@ int qcuBenchCopy(size_t bytes, int iters, double* ms) {
     QREQ(ms && iters > 0 && bytes > 0, "qcuBenchCopy: bad argument");
     DevBuf* a = ctx().buf("copy_a", bytes);
     DevBuf* b = ctx().buf("copy_b", bytes);
+    QREQ(bytes % 16 == 0, "qcuBenchCopy: bytes must be a multiple of 16");
     QHIP(hipMemsetAsync(a->p, 1, bytes, stream()));
-    QHIP(hipMemcpyAsync(b->p, a->p, bytes, hipMemcpyDeviceToDevice, stream()));
+    launch_copy(b->p, a->p, bytes, stream());
     hipEvent_t e0, e1;
     QHIP(hipEventCreate(&e0));
     QHIP(hipEventCreate(&e1));
     QHIP(hipEventRecord(e0, stream()));
-    for (int i = 0; i < iters; ++i)
-      QHIP(hipMemcpyAsync((i & 1) ? a->p : b->p, (i & 1) ? b->p : a->p, bytes, hipMemcpyDeviceToDevice, stream()));
+    for (int i = 0; i < iters; ++i) launch_copy((i & 1) ? a->p : b->p, (i & 1) ? b->p : a->p, bytes, stream());
     QHIP(hipEventRecord(e1, stream()));
     QHIP(hipEventSynchronize(e1));
     float t = 0.f;

@@ -159,6 +159,12 @@ __global__ void cg_xpay_kernel(Vec r, Vec p, const CgScal* sc, long vh) {
   cx<T> rv[12], pv[12];
   SpinorIO<P>::load(static_cast<const char*>(r.p) + k * r.stride, vh, i, rv);
   char* pp = static_cast<char*>(p.p) + k * p.stride;
+  if (beta == T(0)) {
+    // restart (first iteration or converged): p = r without reading p, whose old
+    // contents need not be finite
+    SpinorIO<P>::store(pp, vh, i, rv);
+    return;
+  }
   SpinorIO<P>::load(pp, vh, i, pv);
 #pragma unroll
   for (int c = 0; c < 12; ++c) pv[c] = rv[c] + beta * pv[c];
@@ -197,6 +203,21 @@ __global__ void cg_update_kernel(Vec x, Vec p, Vec t, Vec s, const CgScal* sc, d
     acc = (double)n;
   }
   put_partial(part, vh, k, i, acc);
+}
+
+// plain streaming copy (16 B per lane, grid-stride): the bandwidth and PMC-counter
+// reference kernel
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, long n) {
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  long n = (long)(bytes / 16);
+  long blocks = (n + 255) / 256;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), n);
 }
 
 // ------------------------------------------------------------- launchers ----

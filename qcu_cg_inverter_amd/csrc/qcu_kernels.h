@@ -38,6 +38,8 @@ struct CgScal {
 };
 
 void launch_zero(int prec, Vec v, long vh, int nrhs, hipStream_t st);
+// dst = src, bytes a multiple of 16 (reference kernel for bandwidth and counters)
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 // partials[r][w] = sum over wave w of |v_r|^2
 void launch_norm2(int prec, Vec v, long vh, int nrhs, double* partials, hipStream_t st);
 // out[r] = sum_w partials[r][w] in a fixed order (one block per right-hand side)

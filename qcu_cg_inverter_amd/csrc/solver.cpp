@@ -21,6 +21,8 @@
 // that right-hand side's own numbers, so a solve gives the same bits alone or batched.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "qcu_host.h"
@@ -154,6 +156,7 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
     launch_spinor_convert(F64, shi, P, S, vh, nrhs, 0, st);
     launch_zero(XP, X, vh, nrhs, st);
   }
+  launch_zero(P, Pv, vh, nrhs, st);  // scratch may hold stale, non-finite bits
   QHIP(hipGetLastError());
   norms_host(F64, shi, vh, nrhs, ss0.data());
 
@@ -193,6 +196,8 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
     QHIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
 
+  const char* dbg_env = getenv("QCU_DEBUG");
+  const bool debug = dbg_env && dbg_env[0] && dbg_env[0] != '0';
   // s = b^ - Mhat x in FP64 for right-hand side r; returns |s|^2
   auto reliable_update = [&](int r) {
     if (P != F64) {
@@ -205,6 +210,7 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
     norms_host(F64, at(shi, r), vh, 1, &ss);
     if (P != F64) launch_spinor_convert(F64, at(shi, r), P, at(S, r), vh, 1, 0, st);
     upd[r]++;
+    if (debug) fprintf(stderr, "qcu cg reliable update r=%d |s|^2=%.6e\n", r, ss);
     return ss;
   };
 
@@ -218,6 +224,13 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
     QHIP(hipGetLastError());
     QHIP(hipMemcpyAsync(hsc, dsc, sizeof(CgScal) * nrhs, hipMemcpyDeviceToHost, st));
     QHIP(hipStreamSynchronize(st));
+    if (debug) {
+      for (int r = 0; r < nrhs; ++r) {
+        const CgScal& s = hsc[r];
+        fprintf(stderr, "qcu cg r=%d it=%d ss=%.6e gamma=%.6e tt=%.6e alpha=%.6e beta=%.6e conv=%d act=%d brk=%d\n", r,
+                s.iter, s.ss, s.gamma, s.tt, s.alpha, s.beta, s.converged, s.active, s.breakdown);
+      }
+    }
     bool changed = false;
     any_active = false;
     for (int r = 0; r < nrhs; ++r) {

@@ -84,7 +84,7 @@ def test_clover_matches_oracle(q, ck):
     assert nf == nbad
     cond = np.linalg.cond(Aref.reshape(-1, 6, 6)).max()
     assert np.abs(Ainv - inv_ref).max() < 1e-14 * cond * max(1.0, np.abs(inv_ref).max())
-    assert np.abs(np.einsum("...ij,...jk->...ik", A, Ainv) - np.eye(6)).max() < 1e-12
+    assert np.abs(np.einsum("...ij,...jk->...ik", A, Ainv) - np.eye(6)).max() < 1e-14 * cond
 
 
 @pytest.mark.parametrize("csw", [0.0, 1.0])
