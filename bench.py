@@ -36,6 +36,95 @@ BYTES_PER_SITE = {("f64", 18): 2880, ("f64", 12): 2496, ("f32", 18): 1440, ("f32
                   ("f16", 18): 756, ("f16", 12): 660}
 PREC = {"f64": 0, "f32": 1, "f16": 2}
 L_MAIN = (16, 16, 16, 32)
+LINK_BYTES = {("f64", 18): 144, ("f64", 12): 96, ("f32", 18): 72, ("f32", 12): 48, ("f16", 18): 36, ("f16", 12): 24}
+SPINOR_BYTES = {"f64": 192, "f32": 96, "f16": 52}
+
+
+def hop_bytes(prec, recon, nrhs=1):
+    """Algorithmic bytes per output half-site of one launch over nrhs right-hand
+    sides: 8 links shared by all, then per right-hand side 8 neighbour spinors read
+    and one written."""
+    return 8 * LINK_BYTES[(prec, recon)] + nrhs * 9 * SPINOR_BYTES[prec]
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return r, time.perf_counter() - t0
+
+
+def multi_rhs_section(q):
+    """12-RHS hop (links loaded once per site for all) vs 12 single launches, 32^3x64."""
+    L = (32, 32, 32, 64)
+    g = q.Gauge.random(L, seed=3, eps=0.25)
+    vh = q.half_volume(L)
+    res = {}
+    for prec, recon in (("f64", 18), ("f64", 12), ("f32", 12), ("f16", 12)):
+        shape, _ = q.tune_hop(g, PREC[prec], recon, 12)
+        q.tune_hop(g, PREC[prec], recon, 1)
+        ms12 = q.bench_hop(g, precision=PREC[prec], recon=recon, nrhs=12, iters=20)
+        ms1 = q.bench_hop(g, precision=PREC[prec], recon=recon, nrhs=1, iters=50)
+        gbs = hop_bytes(prec, recon, 12) * vh / (ms12 * 1e-3) / 1e9
+        res[f"{prec}_r{recon}"] = {"us_12rhs": round(ms12 * 1e3, 1), "us_1rhs": round(ms1 * 1e3, 1),
+                                   "speedup_vs_12_single": round(12 * ms1 / ms12, 3),
+                                   "GBps_algorithmic": round(gbs, 1), "bytes_per_site": hop_bytes(prec, recon, 12),
+                                   "shape": shape}
+        log(f"12-rhs hop {prec} r{recon}: {ms12*1e3:.0f} us (12 x single {12*ms1*1e3:.0f} us), {gbs:.0f} GB/s")
+    g.destroy()
+    return res
+
+
+def config3_section(q):
+    """24^3x48 Wilson-clover c_sw=1, kappa=0.122, eps=0.25, 1 Gaussian source, tol 1e-10:
+    FP16 (+ per-site norm) inner with FP64 reliable updates vs plain FP64."""
+    L = (24, 24, 24, 48)
+    g = q.Gauge.random(L, seed=5, eps=0.25)
+    op = q.Operator(0.122, 1.0, -1, 12)
+    b = q.source_gaussian_device(L, seed=6)
+    x = q.DeviceBuffer(2 * q.half_volume(L) * 192)
+    out = {}
+    for name, prec, graph in (("fp64_plain", 0, 0), ("mixed_fp16", 2, 1), ("mixed_fp32", 1, 1)):
+        sp = q.default_solve_params(tol=1e-10, precision=prec, graph=graph)
+        q.solve_batch_device(g, op, [x], [b], sp)  # warm
+        st = q.solve_batch_device(g, op, [x], [b], sp)[0]
+        out[name] = {"seconds": round(st.seconds, 4), "iterations": st.iterations,
+                     "reliable_updates": st.reliable_updates, "true_residual": st.true_residual}
+        log(f"config3 {name}: {st.seconds*1e3:.1f} ms, {st.iterations} it, true res {st.true_residual:.2e}")
+    out["speedup_fp16_vs_fp64"] = round(out["fp64_plain"]["seconds"] / out["mixed_fp16"]["seconds"], 3)
+    g.destroy()
+    return out
+
+
+def config4_section(q):
+    """32^3x64 clover (c_sw=1, kappa=0.122, eps=0.25), 12 point sources (all spin x
+    colour at the origin), tol 1e-10 per RHS: one batched FP32-inner solve with
+    reliable updates and graph-replayed iterations against 12 plain FP64 solves run
+    one at a time (no graph)."""
+    L = (32, 32, 32, 64)
+    g = q.Gauge.random(L, seed=4, eps=0.25)
+    op = q.Operator(0.122, 1.0, -1, 12)
+    vh = q.half_volume(L)
+    srcs = [q.source_point_device(L, (0, 0, 0, 0), s, c) for s in range(4) for c in range(3)]
+    xs = [q.DeviceBuffer(2 * vh * 192) for _ in range(12)]
+    for prec, nrhs in ((1, 12), (0, 1), (1, 1)):
+        q.tune_hop(g, prec, 12, nrhs)
+    sp64 = q.default_solve_params(tol=1e-10, precision=0, graph=0)
+    q.solve_batch_device(g, op, xs[:1], srcs[:1], sp64)  # warm
+    sts64, t64 = timed(lambda: [q.solve_batch_device(g, op, [xs[i]], [srcs[i]], sp64)[0] for i in range(12)])
+    spm = q.default_solve_params(tol=1e-10, precision=1, graph=1)
+    q.solve_batch_device(g, op, xs, srcs, spm)  # warm (tuning, graph capture, buffers)
+    stsm, tm = timed(lambda: q.solve_batch_device(g, op, xs, srcs, spm))
+    out = {"plain_fp64_sequential": {"seconds": round(t64, 3), "solves_per_s": round(12 / t64, 3),
+                                     "iterations": [s.iterations for s in sts64],
+                                     "max_true_residual": max(s.true_residual for s in sts64)},
+           "batched_mixed_fp32": {"seconds": round(tm, 3), "solves_per_s": round(12 / tm, 3),
+                                  "iterations": [s.iterations for s in stsm],
+                                  "reliable_updates": [s.reliable_updates for s in stsm],
+                                  "max_true_residual": max(s.true_residual for s in stsm)}}
+    out["speedup_solves_per_s"] = round(t64 / tm, 3)
+    log(f"config4: plain fp64 12 sequential {t64:.2f} s, batched mixed {tm:.2f} s, x{t64/tm:.2f}")
+    g.destroy()
+    return out
 
 
 def log(*a):
@@ -87,10 +176,37 @@ def cpu_baseline(seconds_target=15.0):
             break
     vh = orc.half_volume(L)
     gbs = BYTES_PER_SITE[("f64", 18)] * vh * n / el / 1e9
+    # the CPU-runnable configuration: 8^4 Wilson, kappa 0.12, eps 0.1, antiperiodic t,
+    # one Gaussian source, FP64 even-odd CG to 1e-10
+    L1 = (8, 8, 8, 8)
+    g1 = orc.gauge_random(L1, 1, 0.1)
+    b1 = orc.source_gauss(L1, 2)
+    t0 = time.perf_counter()
+    _, st = orc.cg_eo(b1, g1, L1, 0.12, tbc=-1.0, recon=18, tol=1e-10)
+    tcg = time.perf_counter() - t0
     del np
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{n} FP64 hop applications on 16^3x32 ({el:.1f} s, oracle, OpenMP)",
-            "ms_per_apply": round(1e3 * el / n, 3)}
+            "ms_per_apply": round(1e3 * el / n, 3),
+            "cg_config1_8x8x8x8": {"seconds": round(tcg, 4), "iterations": st.iterations,
+                                   "true_residual": st.true_residual}}
+
+
+def config1_gpu(q):
+    """8^4 Wilson, kappa 0.12, eps 0.1, antiperiodic t, 1 Gaussian source: plain FP64 GPU
+    solve and the mixed solve, true residual 1e-10."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=1, eps=0.1)
+    b = q.source_gaussian_device(L, seed=2)
+    x = q.DeviceBuffer(2 * q.half_volume(L) * 192)
+    out = {}
+    for name, prec, graph in (("fp64_plain", 0, 0), ("mixed_fp32", 1, 1)):
+        sp = q.default_solve_params(tol=1e-10, precision=prec, graph=graph)
+        q.solve_batch_device(g, q.Operator(0.12, 0.0, -1, 18), [x], [b], sp)
+        st = q.solve_batch_device(g, q.Operator(0.12, 0.0, -1, 18), [x], [b], sp)[0]
+        out[name] = {"seconds": round(st.seconds, 5), "iterations": st.iterations, "true_residual": st.true_residual}
+    g.destroy()
+    return out
 
 
 def main():
@@ -191,6 +307,10 @@ def main():
                     log(f"hop {prec} r{recon} {key}: {ms*1e3:.1f} us, {gbs:.0f} GB/s")
             gl.destroy()
         out["dslash_large"] = large
+        out["config1_wilson_8x8x8x8"] = config1_gpu(q)
+        out["dslash_12rhs"] = multi_rhs_section(q)
+        out["config3_clover_24x48"] = config3_section(q)
+        out["config4_clover_32x64_12rhs"] = config4_section(q)
 
     if not (args.quick or args.no_cpu):
         out["cpu_baseline"] = cpu_baseline()

@@ -97,12 +97,18 @@ int qcuCloverExport(qcuGauge g, const qcuOperator* op, void* A, void* Ainv, int*
 /* ---- sources ---------------------------------------------------------------- */
 /* N(0,1) per real of a full spinor of `lattice_size`, element k from counter k */
 int qcuSourceGaussian(void* v, const int lattice_size[4], uint64_t seed);
+/* unit point source: 1 at full coordinates `coords` (x,y,z,t), spin, colour; 0 elsewhere */
+int qcuSourcePoint(void* v, const int lattice_size[4], const int coords[4], int spin, int colour);
 
 /* ---- operators (interface layouts) --------------------------------------- */
 /* out(parity) = D^s in(1-parity), s = -1 if dagger; precision selects the kernel
  * (inputs are rounded to it); recon 12 or 18 */
 int qcuHop(qcuGauge g, void* out, const void* in, int parity, int dagger, int t_boundary, int recon,
            int precision);
+/* the same for nrhs half spinors in one launch (each link loaded once per site for
+ * all of them); out[r] and in[r] are separate device buffers */
+int qcuHopBatch(qcuGauge g, int nrhs, void* const* out, const void* const* in, int parity, int dagger,
+                int t_boundary, int recon, int precision);
 /* full lattice: out = (A - kappa D_s) in */
 int qcuApplyM(qcuGauge g, const qcuOperator* op, void* out, const void* in, int dagger);
 /* even half spinors: out = Mhat in = A_ee in - kappa^2 D_eo A_oo^-1 D_oe in (s = +1 or
@@ -128,6 +134,15 @@ int qcuSolveBatch(qcuGauge g, const qcuOperator* op, const qcuSolveParams* sp, i
 int qcuBenchHop(qcuGauge g, int precision, int recon, int nrhs, int iters, double* ms);
 /* device copy bandwidth: ms per copy of `bytes` (read + write) */
 int qcuBenchCopy(size_t bytes, int iters, double* ms);
+
+/* ---- tuning ------------------------------------------------------------------ */
+/* Times every launch shape of the hopping kernel for (precision, recon, nrhs) on this
+ * gauge's volume and keeps the fastest for later launches: threads per block,
+ * wave groups per block sharing sites (rb), right-hand sides per thread (rpt),
+ * XCD-contiguous block order (xcd).  Outputs may be NULL.  QCU_TUNE=1 makes solves
+ * tune on first use; QCU_TUNE_CACHE=<file> persists results across processes. */
+int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* block, int* rb, int* rpt, int* xcd,
+               double* ms);
 
 #ifdef __cplusplus
 }

@@ -99,7 +99,9 @@ PROTOTYPES = {
     "qcuGaugeExport": (_int, [_vp, _vp]),
     "qcuCloverExport": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _i4]),
     "qcuSourceGaussian": (_int, [_vp, _i4, ctypes.c_uint64]),
+    "qcuSourcePoint": (_int, [_vp, _i4, _i4, _int, _int]),
     "qcuHop": (_int, [_vp, _vp, _vp, _int, _int, _int, _int, _int]),
+    "qcuHopBatch": (_int, [_vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _int, _int, _int, _int, _int]),
     "qcuApplyM": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int]),
     "qcuApplyMhat": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int, _int]),
     "qcuDefaultSolveParams": (None, [ctypes.POINTER(SolveParams)]),
@@ -110,6 +112,7 @@ PROTOTYPES = {
                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(SolveStats)]),
     "qcuBenchHop": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dbl)]),
     "qcuBenchCopy": (_int, [_sz, _int, ctypes.POINTER(_dbl)]),
+    "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, _i4, _i4, _i4, ctypes.POINTER(_dbl)]),
 }
 
 _lib = None
@@ -255,6 +258,17 @@ def hop(g: Gauge, inp: np.ndarray, parity: int, dagger=0, t_boundary=1, recon=12
     return o.to_numpy((vh, 4, 3))
 
 
+def hop_batch(g: Gauge, inps, parity: int, dagger=0, t_boundary=1, recon=12, precision=PREC_DOUBLE):
+    vh = g.vh
+    n = len(inps)
+    ins = [DeviceBuffer.from_numpy(np.ascontiguousarray(v, np.complex128)) for v in inps]
+    outs = [DeviceBuffer(vh * 192) for _ in range(n)]
+    ip = (_vp * n)(*[b.ptr for b in ins])
+    op_ = (_vp * n)(*[b.ptr for b in outs])
+    check(lib().qcuHopBatch(g.handle, n, op_, ip, parity, dagger, t_boundary, recon, precision))
+    return [o.to_numpy((vh, 4, 3)) for o in outs]
+
+
 def apply_m(g: Gauge, op: Operator, inp: np.ndarray, dagger=0):
     vh = g.vh
     a = DeviceBuffer.from_numpy(np.ascontiguousarray(inp, np.complex128))
@@ -310,6 +324,43 @@ def bench_hop(g: Gauge, precision=PREC_DOUBLE, recon=18, nrhs=1, iters=100) -> f
     ms = ctypes.c_double()
     check(lib().qcuBenchHop(g.handle, precision, recon, nrhs, iters, ctypes.byref(ms)))
     return ms.value
+
+
+def source_point_device(L, coords, spin, colour) -> DeviceBuffer:
+    """Unit point source as a device full spinor (no host copy)."""
+    buf = DeviceBuffer(2 * half_volume(L) * 192)
+    check(lib().qcuSourcePoint(buf.ptr, _L(L), _L(coords), spin, colour))
+    return buf
+
+
+def source_gaussian_device(L, seed) -> DeviceBuffer:
+    buf = DeviceBuffer(2 * half_volume(L) * 192)
+    check(lib().qcuSourceGaussian(buf.ptr, _L(L), seed))
+    return buf
+
+
+def solve_batch_device(g: Gauge, op: Operator, xs, bs, params: SolveParams | None = None, raise_on_error=True):
+    """Batched solve on device buffers (DeviceBuffer lists); returns per-RHS stats."""
+    sp = params or default_solve_params()
+    n = len(bs)
+    xp = (_vp * n)(*[x.ptr for x in xs])
+    bp = (_vp * n)(*[b.ptr for b in bs])
+    st = (SolveStats * n)()
+    if n == 1:
+        rc = lib().qcuSolve(g.handle, ctypes.byref(op), ctypes.byref(sp), xs[0].ptr, bs[0].ptr, ctypes.byref(st[0]))
+    else:
+        rc = lib().qcuSolveBatch(g.handle, ctypes.byref(op), ctypes.byref(sp), n, xp, bp, st)
+    if raise_on_error:
+        check(rc)
+    return list(st)
+
+
+def tune_hop(g: Gauge, precision=PREC_DOUBLE, recon=18, nrhs=1):
+    """Autotune the hopping kernel's launch shape; returns (shape dict, ms)."""
+    v = [ctypes.c_int() for _ in range(4)]
+    ms = ctypes.c_double()
+    check(lib().qcuTuneHop(g.handle, precision, recon, nrhs, *[ctypes.byref(x) for x in v], ctypes.byref(ms)))
+    return dict(zip(("block", "rb", "rpt", "xcd"), [x.value for x in v])), ms.value
 
 
 def bench_copy(nbytes, iters=20) -> float:

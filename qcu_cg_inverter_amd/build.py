@@ -19,15 +19,15 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libqcu.so")
 
 SOURCES = [
-    "k_hop_f64.hip",
-    "k_hop_f32.hip",
-    "k_hop_f16.hip",
+    *[f"k_hop_{p}_{r}.hip" for p in ("f64", "f32", "f16") for r in (1, 2, 3, 4)],
+    "k_hop.hip",
     "k_layout.hip",
     "k_blas.hip",
     "k_clover.hip",
     "k_gen.hip",
     "api.cpp",
     "solver.cpp",
+    "tune.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

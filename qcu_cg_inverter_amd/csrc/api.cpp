@@ -25,6 +25,7 @@ static void record(int code, const std::string& msg) {
   g_last_code = code;
   g_last_msg = msg;
 }
+void set_last_error(int code, const std::string& msg) { record(code, msg); }
 
 template <typename F>
 static int guarded(F&& f) {
@@ -106,6 +107,7 @@ void ensure_init() {
   c.default_solve.precision = QCU_PREC_DOUBLE;
   c.default_solve.tol = 1e-12;
   c.inited = true;
+  load_tune_cache();
 }
 
 void validate_lattice(const int L[4]) {
@@ -207,8 +209,10 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
   auto it = ctx().tune.find(tune_key(prec, recon, vh, nrhs));
   if (it != ctx().tune.end()) return it->second;
   Tune t;
-  if (nrhs > 1) {
+  if (nrhs > 1) {  // wave groups of 64 sites; FP32/FP16 also carry several per thread
     t.rb = nrhs >= 4 ? 4 : nrhs;
+    t.rpt = prec == F64 ? 1 : (nrhs + t.rb - 1) / t.rb;
+    if (t.rpt > 4) t.rpt = 4;
     t.block = 64 * t.rb;
   }
   return t;
@@ -241,7 +245,7 @@ void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, do
   a.nrhs = nrhs;
   Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
   a.xcd_remap = t.xcd;
-  launch_hop(prec, recon, epi, a, t.block, t.rb, st);
+  launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }
 
 }  // namespace qcu
@@ -490,6 +494,24 @@ int qcuSourceGaussian(void* v, const int L[4], uint64_t seed) {
   });
 }
 
+int qcuSourcePoint(void* v, const int L[4], const int coords[4], int spin, int colour) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(v && L && coords, "qcuSourcePoint: NULL argument");
+    validate_lattice(L);
+    QREQ(spin >= 0 && spin < 4 && colour >= 0 && colour < 3, "spin must be 0..3 and colour 0..2");
+    for (int i = 0; i < 4; ++i) QREQ(coords[i] >= 0 && coords[i] < L[i], "coordinate outside the lattice");
+    long vh = (long)L[0] * L[1] * L[2] * L[3] / 2;
+    int parity = (coords[0] + coords[1] + coords[2] + coords[3]) & 1;
+    long idx = (((long)coords[3] * L[2] + coords[2]) * L[1] + coords[1]) * (L[0] / 2) + coords[0] / 2;
+    long off = ((parity * vh + idx) * 12 + spin * 3 + colour) * 16;
+    static const double one[2] = {1.0, 0.0};
+    QHIP(hipMemsetAsync(v, 0, (size_t)2 * vh * 192, stream()));
+    QHIP(hipMemcpyAsync(static_cast<char*>(v) + off, one, 16, hipMemcpyHostToDevice, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
 int qcuHop(qcuGauge h, void* out, const void* in, int parity, int dagger, int t_boundary, int recon, int precision) {
   return guarded([&] {
     ensure_init();
@@ -506,6 +528,31 @@ int qcuHop(qcuGauge h, void* out, const void* in, int parity, int dagger, int t_
     hop(g, precision, recon, EPI_PLAIN, parity, dagger ? 1 : 0, (double)t_boundary, vec(a->p), vec(b->p),
         vec(nullptr), nullptr, 1.0, 1, nullptr, stream());
     launch_spinor_to_aos(precision, vec(b->p), out, 0, vh, 1, stream());
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qcuHopBatch(qcuGauge h, int nrhs, void* const* out, const void* const* in, int parity, int dagger,
+                int t_boundary, int recon, int precision) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && out && in, "qcuHopBatch: NULL argument");
+    QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range (1..16)");
+    QREQ(parity == 0 || parity == 1, "parity must be 0 or 1");
+    QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
+    QREQ(precision >= 0 && precision <= 2, "bad precision");
+    QREQ(t_boundary == 1 || t_boundary == -1, "t_boundary must be +1 or -1");
+    for (int r = 0; r < nrhs; ++r) QREQ(out[r] && in[r], "qcuHopBatch: NULL spinor");
+    GaugeField& g = *reinterpret_cast<GaugeField*>(h);
+    long vh = g.geo.vh;
+    long fb = (long)spinor_bytes(precision, vh);
+    DevBuf* a = ctx().buf("hopb_in", (size_t)fb * nrhs);
+    DevBuf* b = ctx().buf("hopb_out", (size_t)fb * nrhs);
+    for (int r = 0; r < nrhs; ++r) launch_spinor_from_aos(precision, in[r], 0, vec(a->c() + r * fb), vh, 1, stream());
+    hop(g, precision, recon, EPI_PLAIN, parity, dagger ? 1 : 0, (double)t_boundary, vec(a->p, fb), vec(b->p, fb),
+        vec(nullptr), nullptr, 1.0, nrhs, nullptr, stream());
+    for (int r = 0; r < nrhs; ++r) launch_spinor_to_aos(precision, vec(b->c() + r * fb), out[r], 0, vh, 1, stream());
     QHIP(hipGetLastError());
     QHIP(hipStreamSynchronize(stream()));
   });

@@ -112,6 +112,7 @@ struct HopArgs {
   int nrhs;
   int xcd_remap;                         // contiguous site ranges per XCD
   int sg;                                // sites per right-hand side group (set by the launcher)
+  int ngroups;                           // blocks per site range (set by the launcher)
 };
 
 }  // namespace qcu

@@ -63,9 +63,12 @@ struct GaugeField {
   size_t bytes() const;
 };
 
+// launch shape of the hopping kernel: `block` threads in `rb` wave groups that share
+// sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off
 struct Tune {
   int block = 256;
   int rb = 1;
+  int rpt = 1;
   int xcd = 1;
 };
 
@@ -95,6 +98,12 @@ std::unique_ptr<GaugeField> make_gauge_random(const int L[4], uint64_t seed, dou
 
 Tune tuned_hop(int prec, int recon, long vh, int nrhs);
 void set_tuned_hop(int prec, int recon, long vh, int nrhs, const Tune& t);
+// autotuner (tune.cpp)
+void load_tune_cache();
+Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms);
+void maybe_autotune(GaugeField& g, int prec, int recon, int nrhs);
+
+void set_last_error(int code, const std::string& msg);
 
 // one hop launch on nrhs strided fields
 void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,

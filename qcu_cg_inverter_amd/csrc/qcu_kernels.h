@@ -17,7 +17,9 @@ inline Vec vec(void* p, long stride = 0) { return Vec{p, stride}; }
 __host__ __device__ inline long n_partials(long vh) { return (vh + 63) / 64; }
 
 // ------------------------------------------------------------------ hop -----
-void launch_hop(int prec, int recon, int epi, const HopArgs& a, int block, int rb, hipStream_t st);
+// block threads = sg * rb; rb groups of sg sites share links; rpt right-hand sides
+// per thread (1..4)
+void launch_hop(int prec, int recon, int epi, const HopArgs& a, int block, int rb, int rpt, hipStream_t st);
 
 // --------------------------------------------------------------- layout -----
 // interface spinor (complex128 [Vh][4][3]) <-> internal layout of `prec`

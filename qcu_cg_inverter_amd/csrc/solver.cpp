@@ -114,6 +114,7 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
   if (sp.max_iter <= 0) sp.max_iter = (int)std::min<long>(10 * 2 * vh, 1000000);
   OpView o = make_view(g, op, P);
   if (P != F64 && o.clover) g.clover(op.kappa * op.csw, F64);
+  maybe_autotune(g, P, op.recon, nrhs);  // QCU_TUNE=1: before any graph is captured
   hipStream_t st = stream();
 
   const long fb = (long)spinor_bytes(F64, vh), pb = (long)spinor_bytes(P, vh), xb = (long)spinor_bytes(XP, vh);

@@ -1,0 +1,158 @@
+// tune.cpp -- autotuner of the hopping kernel's launch shape.
+//
+// For one (precision, link storage, half volume, right-hand sides) it times every
+// candidate shape -- block size, wave groups per block sharing sites, right-hand
+// sides per thread, XCD-contiguous block order -- on scratch fields with hipEvents
+// and keeps the fastest.  Results live in the context's table and, when
+// QCU_TUNE_CACHE names a file, are appended there and reloaded by later processes.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "qcu_host.h"
+
+namespace qcu {
+
+static const char* cache_file() {
+  const char* p = getenv("QCU_TUNE_CACHE");
+  return (p && *p) ? p : nullptr;
+}
+
+static bool cache_loaded = false;
+
+void load_tune_cache() {
+  if (cache_loaded) return;
+  cache_loaded = true;
+  const char* path = cache_file();
+  if (!path) return;
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  char key[128];
+  Tune t;
+  double ms;
+  while (fscanf(f, "%127s %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &ms) == 6) ctx().tune[key] = t;
+  fclose(f);
+}
+
+static void append_cache(const std::string& key, const Tune& t, double ms) {
+  const char* path = cache_file();
+  if (!path) return;
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, ms);
+  fclose(f);
+}
+
+static std::vector<Tune> candidates(int prec, int nrhs) {
+  std::vector<Tune> out;
+  const int blocks[] = {64, 128, 256};
+  for (int xcd = 0; xcd <= 1; ++xcd)
+    for (int block : blocks)
+      for (int rb = 1; rb <= 4; rb *= 2) {
+        if (block % (64 * rb) != 0 || rb > nrhs) continue;
+        for (int rpt = 1; rpt <= 4; ++rpt) {
+          if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
+          if (prec == F64 && rpt > 1) continue;  // FP64 spills beyond one right-hand side per thread
+          Tune t;
+          t.block = block;
+          t.rb = rb;
+          t.rpt = rpt;
+          t.xcd = xcd;
+          out.push_back(t);
+        }
+      }
+  return out;
+}
+
+static std::string key_of(int prec, int recon, long vh, int nrhs) {
+  char b[96];
+  snprintf(b, sizeof b, "hop:%d:%d:%ld:%d", prec, recon, vh, nrhs);
+  return b;
+}
+
+Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms) {
+  load_tune_cache();
+  const long vh = g.geo.vh;
+  const size_t fb = spinor_bytes(prec, vh);
+  DevBuf* src = ctx().buf("tune_src", (size_t)vh * 192 * nrhs);
+  DevBuf* a = ctx().buf("tune_a", fb * nrhs);
+  DevBuf* b = ctx().buf("tune_b", fb * nrhs);
+  launch_source_gauss(src->p, vh * 12 * nrhs, 777, stream());
+  launch_spinor_from_aos(prec, src->p, vh * 192, vec(a->p, (long)fb), vh, nrhs, stream());
+  const void* gauge = g.get(prec, recon);
+  HopArgs args;
+  memset(&args, 0, sizeof args);
+  args.in = a->p;
+  args.out = b->p;
+  args.in_stride = args.out_stride = (long)fb;
+  args.gauge = gauge;
+  for (int r = 0; r < nrhs; ++r) args.coef[r] = 1.0;
+  args.geo = g.geo;
+  args.tbc = -1.0;
+  args.nrhs = nrhs;
+  hipEvent_t e0, e1;
+  QHIP(hipEventCreate(&e0));
+  QHIP(hipEventCreate(&e1));
+  Tune best;
+  double bms = 1e30;
+  const int iters = vh * nrhs > (1 << 22) ? 10 : 30;
+  for (const Tune& t : candidates(prec, nrhs)) {
+    args.xcd_remap = t.xcd;
+    for (int w = 0; w < 2; ++w) launch_hop(prec, recon, EPI_PLAIN, args, t.block, t.rb, t.rpt, stream());
+    QHIP(hipEventRecord(e0, stream()));
+    for (int i = 0; i < iters; ++i) {
+      args.parity = i & 1;
+      launch_hop(prec, recon, EPI_PLAIN, args, t.block, t.rb, t.rpt, stream());
+    }
+    QHIP(hipEventRecord(e1, stream()));
+    QHIP(hipEventSynchronize(e1));
+    QHIP(hipGetLastError());
+    float ms = 0.f;
+    QHIP(hipEventElapsedTime(&ms, e0, e1));
+    double per = (double)ms / iters;
+    if (per < bms) {
+      bms = per;
+      best = t;
+    }
+  }
+  QHIP(hipEventDestroy(e0));
+  QHIP(hipEventDestroy(e1));
+  std::string key = key_of(prec, recon, vh, nrhs);
+  ctx().tune[key] = best;
+  append_cache(key, best, bms);
+  if (best_ms) *best_ms = bms;
+  return best;
+}
+
+// tune once per key when QCU_TUNE=1 (used by the solver before capturing graphs)
+void maybe_autotune(GaugeField& g, int prec, int recon, int nrhs) {
+  load_tune_cache();
+  const char* e = getenv("QCU_TUNE");
+  if (!(e && strcmp(e, "0") != 0)) return;
+  if (ctx().tune.count(key_of(prec, recon, g.geo.vh, nrhs))) return;
+  autotune_hop(g, prec, recon, nrhs, nullptr);
+}
+
+}  // namespace qcu
+
+extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* block, int* rb, int* rpt, int* xcd,
+                          double* ms) {
+  using namespace qcu;
+  try {
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    ensure_init();
+    QREQ(h, "qcuTuneHop: NULL gauge");
+    QREQ(precision >= 0 && precision <= 2, "bad precision");
+    QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
+    QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
+    Tune t = autotune_hop(*reinterpret_cast<GaugeField*>(h), precision, recon, nrhs, ms);
+    if (block) *block = t.block;
+    if (rb) *rb = t.rb;
+    if (rpt) *rpt = t.rpt;
+    if (xcd) *xcd = t.xcd;
+    return QCU_SUCCESS;
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    return e.code;
+  }
+}

@@ -253,3 +253,49 @@ def test_error_paths(q):
     x, st = q.solve(g, op, b, q.default_solve_params(tol=1e-12, max_iter=3), raise_on_error=False)
     assert not st.converged and st.iterations <= 8
     assert q.lib().qcuGetLastError() == 4
+
+
+# --------------------------------------------------------------- batching ---
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("nrhs", [2, 5, 12, 16])
+def test_hop_batch_bitwise_equals_single(q, prec, nrhs):
+    """One launch over nrhs right-hand sides (shared link loads, several per thread)
+    gives the same bits as single launches, so results never depend on the batch."""
+    L = (8, 4, 8, 8)
+    g = q.Gauge.random(L, seed=31, eps=0.3)
+    vs = [orc.source_gauss(L, 50 + r)[1] for r in range(nrhs)]
+    outs = q.hop_batch(g, vs, 0, dagger=1, t_boundary=-1, recon=12, precision=prec)
+    for r in range(nrhs):
+        single = q.hop(g, vs[r], 0, dagger=1, t_boundary=-1, recon=12, precision=prec)
+        assert np.array_equal(outs[r], single), r
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tuned_shape_keeps_bits(q, prec):
+    L = (8, 8, 8, 16)
+    g = q.Gauge.random(L, seed=32, eps=0.3)
+    vs = [orc.source_gauss(L, 60 + r)[0] for r in range(12)]
+    before = q.hop_batch(g, vs, 1, recon=18, precision=prec)
+    shape, ms = q.tune_hop(g, prec, 18, 12)
+    assert ms > 0 and shape["block"] in (64, 128, 256)
+    after = q.hop_batch(g, vs, 1, recon=18, precision=prec)
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+
+
+def test_batch12_clover_mixed(q):
+    """12 right-hand sides in one batched FP32-inner clover solve (several per
+    thread): every one converges to 1e-10 and equals its own solve alone."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=33, eps=0.25)
+    op = q.Operator(0.122, 1.0, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=1)
+    bs = [orc.source_point(L, (0, 0, 0, 0), s, c) for s in range(4) for c in range(3)]
+    xs, sts = q.solve_batch(g, op, bs, sp)
+    gref = g.to_numpy()
+    A = orc.clover_fields(gref, L, 0.122, 1.0, recon=12)[0]
+    for i in (0, 5, 11):
+        r = bs[i] - orc.full_op(xs[i], gref, L, 0.122, A=A, s=1, tbc=-1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(bs[i]) <= 1.01e-10
+        xa, sa = q.solve(g, op, bs[i], sp)
+        assert np.array_equal(xa, xs[i]) and sa.iterations == sts[i].iterations
