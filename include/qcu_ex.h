@@ -137,12 +137,12 @@ int qcuBenchCopy(size_t bytes, int iters, double* ms);
 
 /* ---- tuning ------------------------------------------------------------------ */
 /* Times every launch shape of the hopping kernel for (precision, recon, nrhs) on this
- * gauge's volume and keeps the fastest for later launches: threads per block,
- * wave groups per block sharing sites (rb), right-hand sides per thread (rpt),
- * XCD-contiguous block order (xcd).  Outputs may be NULL.  QCU_TUNE=1 makes solves
- * tune on first use; QCU_TUNE_CACHE=<file> persists results across processes. */
-int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* block, int* rb, int* rpt, int* xcd,
-               double* ms);
+ * gauge's volume and keeps the fastest for later launches.  shape (may be NULL)
+ * receives 6 ints: threads per block, wave groups per block sharing sites, right-hand
+ * sides per thread, XCD-contiguous block order (0/1), and the z and t extents of the
+ * site-walk tile (0 = storage order).  QCU_TUNE=1 makes solves tune on first use;
+ * QCU_TUNE_CACHE=<file> persists results across processes. */
+int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* shape, double* ms);
 
 #ifdef __cplusplus
 }

@@ -112,7 +112,7 @@ PROTOTYPES = {
                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(SolveStats)]),
     "qcuBenchHop": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dbl)]),
     "qcuBenchCopy": (_int, [_sz, _int, ctypes.POINTER(_dbl)]),
-    "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, _i4, _i4, _i4, ctypes.POINTER(_dbl)]),
+    "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, ctypes.POINTER(_dbl)]),
 }
 
 _lib = None
@@ -357,10 +357,10 @@ def solve_batch_device(g: Gauge, op: Operator, xs, bs, params: SolveParams | Non
 
 def tune_hop(g: Gauge, precision=PREC_DOUBLE, recon=18, nrhs=1):
     """Autotune the hopping kernel's launch shape; returns (shape dict, ms)."""
-    v = [ctypes.c_int() for _ in range(4)]
+    v = (ctypes.c_int * 6)()
     ms = ctypes.c_double()
-    check(lib().qcuTuneHop(g.handle, precision, recon, nrhs, *[ctypes.byref(x) for x in v], ctypes.byref(ms)))
-    return dict(zip(("block", "rb", "rpt", "xcd"), [x.value for x in v])), ms.value
+    check(lib().qcuTuneHop(g.handle, precision, recon, nrhs, v, ctypes.byref(ms)))
+    return dict(zip(("block", "rb", "rpt", "xcd", "tile_z", "tile_t"), list(v))), ms.value
 
 
 def bench_copy(nbytes, iters=20) -> float:

@@ -214,6 +214,7 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
     t.rpt = prec == F64 ? 1 : (nrhs + t.rb - 1) / t.rb;
     if (t.rpt > 4) t.rpt = 4;
     t.block = 64 * t.rb;
+    t.tz = t.tt = 4;  // short neighbour reuse distances (ignored where tiles do not fit)
   }
   return t;
 }
@@ -245,6 +246,8 @@ void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, do
   a.nrhs = nrhs;
   Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
   a.xcd_remap = t.xcd;
+  a.walk = make_walk(g.geo, t.tz, t.tt);
+  a.clov_shared = 1;
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }
 

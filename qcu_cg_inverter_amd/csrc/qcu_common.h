@@ -56,6 +56,17 @@ struct Geom {
   FastDiv dx, dy, dz;
 };
 
+// Order in which the hopping kernel walks the sites.  tile_z = 0: storage order.
+// Otherwise (z, t) is cut into tile_z x tile_t tiles walked one after another, and
+// inside a tile the xy-planes run with t fastest, so a site's t-neighbours are one
+// plane away and its z-neighbours tile_t planes away (instead of a whole t-slice):
+// the neighbour spinors of a batch of right-hand sides stay in the XCD's L2.  A wave
+// still covers 64 consecutive storage sites (the plane size is a multiple of 64).
+struct Walk {
+  int tile_z, tile_t, ntz;
+  FastDiv dplane, dtt, dtz, dntz;
+};
+
 inline Geom make_geom(const int L[4]) {
   Geom g;
   for (int i = 0; i < 4; ++i) g.L[i] = L[i];
@@ -113,6 +124,25 @@ struct HopArgs {
   int xcd_remap;                         // contiguous site ranges per XCD
   int sg;                                // sites per right-hand side group (set by the launcher)
   int ngroups;                           // blocks per site range (set by the launcher)
+  int clov_shared;                       // every right-hand side uses clov[0]
+  Walk walk;                             // site traversal order
 };
+
+// walk with (tile_z, tile_t) tiles, or storage order when the tiles do not divide the
+// lattice or a plane is not a multiple of 64 sites
+inline Walk make_walk(const Geom& g, int tz, int tt) {
+  Walk w;
+  w.tile_z = w.tile_t = w.ntz = 0;
+  long plane = (long)g.lxh * g.L[1];
+  if (tz <= 0 || tt <= 0 || plane % 64 != 0 || g.L[2] % tz != 0 || g.L[3] % tt != 0) return w;
+  w.tile_z = tz;
+  w.tile_t = tt;
+  w.ntz = g.L[2] / tz;
+  w.dplane = FastDiv((uint32_t)plane);
+  w.dtt = FastDiv((uint32_t)tt);
+  w.dtz = FastDiv((uint32_t)tz);
+  w.dntz = FastDiv((uint32_t)w.ntz);
+  return w;
+}
 
 }  // namespace qcu

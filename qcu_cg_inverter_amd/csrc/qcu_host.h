@@ -64,12 +64,15 @@ struct GaugeField {
 };
 
 // launch shape of the hopping kernel: `block` threads in `rb` wave groups that share
-// sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off
+// sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off, and
+// the (z, t) tile of the site walk (0 = storage order)
 struct Tune {
   int block = 256;
   int rb = 1;
   int rpt = 1;
   int xcd = 1;
+  int tz = 0;
+  int tt = 0;
 };
 
 struct Context {

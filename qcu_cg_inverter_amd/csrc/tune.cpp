@@ -30,7 +30,8 @@ void load_tune_cache() {
   char key[128];
   Tune t;
   double ms;
-  while (fscanf(f, "%127s %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &ms) == 6) ctx().tune[key] = t;
+  while (fscanf(f, "%127s %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt, &ms) == 8)
+    ctx().tune[key] = t;
   fclose(f);
 }
 
@@ -39,28 +40,32 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, ms);
+  fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, ms);
   fclose(f);
 }
 
 static std::vector<Tune> candidates(int prec, int nrhs) {
   std::vector<Tune> out;
   const int blocks[] = {64, 128, 256};
-  for (int xcd = 0; xcd <= 1; ++xcd)
-    for (int block : blocks)
-      for (int rb = 1; rb <= 4; rb *= 2) {
-        if (block % (64 * rb) != 0 || rb > nrhs) continue;
-        for (int rpt = 1; rpt <= 4; ++rpt) {
-          if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
-          if (prec == F64 && rpt > 1) continue;  // FP64 spills beyond one right-hand side per thread
-          Tune t;
-          t.block = block;
-          t.rb = rb;
-          t.rpt = rpt;
-          t.xcd = xcd;
-          out.push_back(t);
+  const int tiles[][2] = {{0, 0}, {4, 4}, {8, 8}};
+  for (auto& tl : tiles)
+    for (int xcd = 0; xcd <= 1; ++xcd)
+      for (int block : blocks)
+        for (int rb = 1; rb <= 4; rb *= 2) {
+          if (block % (64 * rb) != 0 || rb > nrhs) continue;
+          for (int rpt = 1; rpt <= 4; ++rpt) {
+            if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
+            if (prec == F64 && rpt > 1) continue;  // FP64 spills beyond one right-hand side per thread
+            Tune t;
+            t.block = block;
+            t.rb = rb;
+            t.rpt = rpt;
+            t.xcd = xcd;
+            t.tz = tl[0];
+            t.tt = tl[1];
+            out.push_back(t);
+          }
         }
-      }
   return out;
 }
 
@@ -98,6 +103,9 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   const int iters = vh * nrhs > (1 << 22) ? 10 : 30;
   for (const Tune& t : candidates(prec, nrhs)) {
     args.xcd_remap = t.xcd;
+    args.walk = make_walk(g.geo, t.tz, t.tt);
+    if (t.tz && !args.walk.tile_z) continue;  // tile does not fit this lattice
+    args.clov_shared = 1;
     for (int w = 0; w < 2; ++w) launch_hop(prec, recon, EPI_PLAIN, args, t.block, t.rb, t.rpt, stream());
     QHIP(hipEventRecord(e0, stream()));
     for (int i = 0; i < iters; ++i) {
@@ -135,8 +143,7 @@ void maybe_autotune(GaugeField& g, int prec, int recon, int nrhs) {
 
 }  // namespace qcu
 
-extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* block, int* rb, int* rpt, int* xcd,
-                          double* ms) {
+extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* shape, double* ms) {
   using namespace qcu;
   try {
     std::lock_guard<std::recursive_mutex> lk(ctx().mu);
@@ -146,10 +153,14 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* b
     QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
     QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
     Tune t = autotune_hop(*reinterpret_cast<GaugeField*>(h), precision, recon, nrhs, ms);
-    if (block) *block = t.block;
-    if (rb) *rb = t.rb;
-    if (rpt) *rpt = t.rpt;
-    if (xcd) *xcd = t.xcd;
+    if (shape) {
+      shape[0] = t.block;
+      shape[1] = t.rb;
+      shape[2] = t.rpt;
+      shape[3] = t.xcd;
+      shape[4] = t.tz;
+      shape[5] = t.tt;
+    }
     return QCU_SUCCESS;
   } catch (const Error& e) {
     set_last_error(e.code, e.what());

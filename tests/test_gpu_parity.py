@@ -258,10 +258,11 @@ def test_error_paths(q):
 # --------------------------------------------------------------- batching ---
 @pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("nrhs", [2, 5, 12, 16])
-def test_hop_batch_bitwise_equals_single(q, prec, nrhs):
-    """One launch over nrhs right-hand sides (shared link loads, several per thread)
-    gives the same bits as single launches, so results never depend on the batch."""
-    L = (8, 4, 8, 8)
+@pytest.mark.parametrize("L", [(8, 4, 8, 8), (16, 8, 8, 8)])
+def test_hop_batch_bitwise_equals_single(q, prec, nrhs, L):
+    """One launch over nrhs right-hand sides (shared link loads, several per thread,
+    tiled site walk where the lattice allows it) gives the same bits as single
+    launches, so results never depend on the batch."""
     g = q.Gauge.random(L, seed=31, eps=0.3)
     vs = [orc.source_gauss(L, 50 + r)[1] for r in range(nrhs)]
     outs = q.hop_batch(g, vs, 0, dagger=1, t_boundary=-1, recon=12, precision=prec)
