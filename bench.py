@@ -95,6 +95,111 @@ def config3_section(q):
     return out
 
 
+def serving_section(q, n_requests=240, n_baseline=12, seed=2024):
+    """Poisson stream of solve requests over 8 resident gauges (2 per volume in
+    {8^4, 16^3x32, 24^3x48, 32^3x64}); each request draws kappa ~ U[0.11, 0.124],
+    Wilson or clover (c_sw = 1.0), 1-12 Gaussian right-hand sides and tol in
+    {1e-8, 1e-10}.  Capacity is measured with a closed burst, arrivals then run at
+    0.8x capacity through the continuous-batching server (FP32 inner, FP64 reliable
+    updates).  The baseline is the plain FP64 solve, one right-hand side at a time,
+    timed on a sample of the same requests and replayed through a FIFO queue with the
+    same arrival rate."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    vols = [(8, 8, 8, 8), (16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
+    gauges = [q.Gauge.random(vols[i // 2], seed=900 + i, eps=0.2 + 0.05 * (i % 2)) for i in range(8)]
+
+    def draw(n):
+        out = []
+        for _ in range(n):
+            gi = int(rng.integers(8))
+            out.append({"g": gi, "kappa": float(rng.uniform(0.11, 0.124)), "csw": 1.0 if rng.random() < 0.5 else 0.0,
+                        "nrhs": int(rng.integers(1, 13)), "tol": 1e-8 if rng.random() < 0.5 else 1e-10,
+                        "seed": int(rng.integers(1 << 40))})
+        return out
+
+    def submit(server, r):
+        return server.submit(gauges[r["g"]], r["kappa"], r["csw"], r["tol"], r["nrhs"], seed=r["seed"])
+
+    # warm-up: build packed gauges, tune the common shapes
+    for gi in range(8):
+        q.tune_hop(gauges[gi], 1, 12, 12)
+        q.tune_hop(gauges[gi], 1, 12, 1)
+    server = q.Server(precision=1, max_batch=12, chunk=8)
+    for r in draw(8):
+        submit(server, r)
+    server.drain()
+    # capacity: closed burst
+    burst = draw(48)
+    t0 = time.perf_counter()
+    for r in burst:
+        submit(server, r)
+    server.drain()
+    tb = time.perf_counter() - t0
+    cap = len(burst) / tb
+    rate = 0.8 * cap
+    # open Poisson stream at 0.8x capacity
+    reqs = draw(n_requests)
+    gaps = rng.exponential(1.0 / rate, size=n_requests)
+    arrivals = np.cumsum(gaps)
+    ids = []
+    t0 = time.perf_counter()
+    for r, ta in zip(reqs, arrivals):
+        d = ta - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        ids.append(submit(server, r))
+    server.drain()
+    tend = time.perf_counter() - t0
+    sts = [server.poll(i) for i in ids]
+    usage = server.usage()
+    server.destroy()
+    lat = np.array([s.latency_seconds for s in sts]) * 1e3
+    nrhs_total = sum(r["nrhs"] for r in reqs)
+    out = {"requests": n_requests, "rhs": nrhs_total, "arrival_rate_per_s": round(rate, 2),
+           "capacity_req_per_s": round(cap, 2), "wall_s": round(tend, 2),
+           "solves_per_s": round(nrhs_total / tend, 2), "requests_per_s": round(n_requests / tend, 2),
+           "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+           "all_converged": all(s.converged for s in sts),
+           "max_true_residual_over_tol": max(s.true_residual / r["tol"] for s, r in zip(sts, reqs)),
+           "peak_device_bytes": usage["peak_bytes"]}
+    # baseline: plain FP64, one right-hand side at a time, on a sample of the stream
+    sample = reqs[:n_baseline]
+    service = []
+    sp = q.default_solve_params(tol=1e-10, precision=0, graph=0)
+    for r in sample:
+        L = vols[r["g"] // 2]
+        op = q.Operator(r["kappa"], r["csw"], -1, 12)
+        sp.tol = r["tol"]
+        x = q.DeviceBuffer(2 * q.half_volume(L) * 192)
+        tsum = 0.0
+        for k in range(r["nrhs"]):
+            b = q.source_gaussian_device(L, r["seed"] + k)
+            t1 = time.perf_counter()
+            q.solve_batch_device(gauges[r["g"]], op, [x], [b], sp)
+            tsum += time.perf_counter() - t1
+            b.free()
+        service.append(tsum)
+    busy_until, blat = 0.0, []
+    for ta, s in zip(arrivals[:n_baseline], service):
+        start = max(ta, busy_until)
+        busy_until = start + s
+        blat.append(busy_until - ta)
+    blat = np.array(blat) * 1e3
+    brhs = sum(r["nrhs"] for r in sample)
+    out["baseline_fp64_one_at_a_time"] = {
+        "sample_requests": n_baseline, "solves_per_s": round(brhs / sum(service), 2),
+        "p50_ms": round(float(np.percentile(blat, 50)), 2), "p99_ms": round(float(np.percentile(blat, 99)), 2),
+        "note": "FIFO replay of measured service times at the same arrival rate"}
+    out["speedup_solves_per_s"] = round(out["solves_per_s"] / out["baseline_fp64_one_at_a_time"]["solves_per_s"], 3)
+    log(f"serving: {out['solves_per_s']} solves/s p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms; "
+        f"fp64 one-at-a-time {out['baseline_fp64_one_at_a_time']['solves_per_s']} solves/s")
+    for g in gauges:
+        g.destroy()
+    return out
+
+
 def config4_section(q):
     """32^3x64 clover (c_sw=1, kappa=0.122, eps=0.25), 12 point sources (all spin x
     colour at the origin), tol 1e-10 per RHS: one batched FP32-inner solve with
@@ -216,6 +321,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--quick", action="store_true", help="skip the large lattices and the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=240,
+                    help="requests in the Poisson serving stream (2000 for the full configuration; 0 skips)")
+    ap.add_argument("--serving-baseline", type=int, default=12,
+                    help="requests of the stream re-run as plain FP64 one-at-a-time solves")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiler driver: 1 GiB copy kernel x5 then the 16^3x32 FP64 hop x steps, nothing else")
     args = ap.parse_args()
@@ -311,6 +420,8 @@ def main():
         out["dslash_12rhs"] = multi_rhs_section(q)
         out["config3_clover_24x48"] = config3_section(q)
         out["config4_clover_32x64_12rhs"] = config4_section(q)
+        if args.serving_requests > 0:
+            out["config5_serving"] = serving_section(q, args.serving_requests, args.serving_baseline)
 
     if not (args.quick or args.no_cpu):
         out["cpu_baseline"] = cpu_baseline()

@@ -135,6 +135,56 @@ int qcuBenchHop(qcuGauge g, int precision, int recon, int nrhs, int iters, doubl
 /* device copy bandwidth: ms per copy of `bytes` (read + write) */
 int qcuBenchCopy(size_t bytes, int iters, double* ms);
 
+/* ---- serving ------------------------------------------------------------------ */
+/* A request loop with continuous batching.  A worker thread keeps one batch of
+ * right-hand-side slots per (gauge, Wilson or clover) on its own stream; requests
+ * that arrive while others run are admitted as slots and the HBM budget free up,
+ * right-hand sides of different requests (and different kappa) on one gauge share
+ * each link load, and batches on different gauges overlap on the device.  A
+ * right-hand side gives the same bits alone or in any mix. */
+typedef struct qcuServer_st* qcuServer;
+
+typedef struct {
+  size_t hbm_budget;  /* device bytes for batches and clover fields (0: 3/4 of free memory) */
+  int max_batch;      /* right-hand-side slots per batch, 1..16 */
+  int precision;      /* inner precision of every solve (QCU_PREC_*) */
+  int chunk;          /* CG iterations per scheduling round */
+  int graph;          /* replay captured hipGraphs */
+} qcuServerConfig;
+
+typedef struct {
+  qcuGauge gauge;          /* resident gauge (must outlive the request) */
+  double kappa, csw, tol;  /* csw = 0: Wilson */
+  int t_boundary, recon;
+  int nrhs;                /* 1..16 */
+  void* const* x;          /* nrhs solution buffers (full spinors) or NULL: residuals only */
+  const void* const* b;    /* nrhs sources or NULL: Gaussian sources, seeds seed + r */
+  uint64_t seed;
+  int max_iter;            /* <= 0: 10 * volume */
+} qcuRequest;
+
+typedef struct {
+  int done;                /* every right-hand side finished */
+  int converged;           /* every right-hand side reached tol */
+  int iterations;          /* max over right-hand sides */
+  double true_residual;    /* max over right-hand sides, FP64 */
+  double queue_seconds;    /* submit -> first right-hand side admitted */
+  double latency_seconds;  /* submit -> last right-hand side finished */
+} qcuRequestStatus;
+
+void qcuDefaultServerConfig(qcuServerConfig* cfg);
+int qcuServerCreate(qcuServer* s, const qcuServerConfig* cfg);
+/* non-blocking; *id identifies the request */
+int qcuServerSubmit(qcuServer s, const qcuRequest* req, uint64_t* id);
+int qcuServerPoll(qcuServer s, uint64_t id, qcuRequestStatus* st);
+/* blocks until request id is done */
+int qcuServerWait(qcuServer s, uint64_t id, qcuRequestStatus* st);
+/* blocks until every submitted request is done */
+int qcuServerDrain(qcuServer s);
+/* device bytes currently held, peak, and batches in flight */
+int qcuServerUsage(qcuServer s, size_t* bytes, size_t* peak_bytes, int* batches);
+int qcuServerDestroy(qcuServer s);
+
 /* ---- tuning ------------------------------------------------------------------ */
 /* Times every launch shape of the hopping kernel for (precision, recon, nrhs) on this
  * gauge's volume and keeps the fastest for later launches.  shape (may be NULL)

@@ -66,6 +66,27 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch", ctypes.c_int), ("precision", ctypes.c_int),
+                ("chunk", ctypes.c_int), ("graph", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("gauge", ctypes.c_void_p), ("kappa", ctypes.c_double), ("csw", ctypes.c_double),
+                ("tol", ctypes.c_double), ("t_boundary", ctypes.c_int), ("recon", ctypes.c_int),
+                ("nrhs", ctypes.c_int), ("x", ctypes.POINTER(ctypes.c_void_p)), ("b", ctypes.POINTER(ctypes.c_void_p)),
+                ("seed", ctypes.c_uint64), ("max_iter", ctypes.c_int)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("done", ctypes.c_int), ("converged", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("true_residual", ctypes.c_double), ("queue_seconds", ctypes.c_double),
+                ("latency_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _vp = ctypes.c_void_p
 _int = ctypes.c_int
 _dbl = ctypes.c_double
@@ -113,6 +134,14 @@ PROTOTYPES = {
     "qcuBenchHop": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dbl)]),
     "qcuBenchCopy": (_int, [_sz, _int, ctypes.POINTER(_dbl)]),
     "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, ctypes.POINTER(_dbl)]),
+    "qcuDefaultServerConfig": (None, [ctypes.POINTER(ServerConfig)]),
+    "qcuServerCreate": (_int, [ctypes.POINTER(_vp), ctypes.POINTER(ServerConfig)]),
+    "qcuServerSubmit": (_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    "qcuServerPoll": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]),
+    "qcuServerWait": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]),
+    "qcuServerDrain": (_int, [_vp]),
+    "qcuServerUsage": (_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), _i4]),
+    "qcuServerDestroy": (_int, [_vp]),
 }
 
 _lib = None
@@ -381,3 +410,66 @@ def grid(g=(1, 1, 1, 1)) -> QcuGrid:
     for i in range(4):
         q.grid_size[i] = int(g[i])
     return q
+
+
+class Server:
+    """Continuous-batching solve server (qcuServer*)."""
+
+    def __init__(self, precision=PREC_SINGLE, max_batch=12, chunk=8, graph=1, hbm_budget=0):
+        cfg = ServerConfig()
+        lib().qcuDefaultServerConfig(ctypes.byref(cfg))
+        cfg.precision, cfg.max_batch, cfg.chunk, cfg.graph, cfg.hbm_budget = precision, max_batch, chunk, graph, hbm_budget
+        h = _vp()
+        check(lib().qcuServerCreate(ctypes.byref(h), ctypes.byref(cfg)))
+        self.handle = h.value
+        self._keep = {}
+
+    def submit(self, gauge: Gauge, kappa, csw, tol, nrhs, seed=0, xs=None, bs=None, t_boundary=-1, recon=12,
+               max_iter=0) -> int:
+        r = Request()
+        r.gauge, r.kappa, r.csw, r.tol = gauge.handle, kappa, csw, tol
+        r.t_boundary, r.recon, r.nrhs, r.seed, r.max_iter = t_boundary, recon, nrhs, seed, max_iter
+        keep = []
+        if xs is not None:
+            arr = (_vp * nrhs)(*[x.ptr for x in xs])
+            r.x = ctypes.cast(arr, ctypes.POINTER(_vp))
+            keep.append(arr)
+        if bs is not None:
+            arr = (_vp * nrhs)(*[b.ptr for b in bs])
+            r.b = ctypes.cast(arr, ctypes.POINTER(_vp))
+            keep.append(arr)
+        rid = ctypes.c_uint64()
+        check(lib().qcuServerSubmit(self.handle, ctypes.byref(r), ctypes.byref(rid)))
+        self._keep[rid.value] = keep
+        return rid.value
+
+    def poll(self, rid) -> RequestStatus:
+        st = RequestStatus()
+        check(lib().qcuServerPoll(self.handle, rid, ctypes.byref(st)))
+        return st
+
+    def wait(self, rid, raise_on_error=True) -> RequestStatus:
+        st = RequestStatus()
+        rc = lib().qcuServerWait(self.handle, rid, ctypes.byref(st))
+        if raise_on_error:
+            check(rc)
+        return st
+
+    def drain(self):
+        check(lib().qcuServerDrain(self.handle))
+
+    def usage(self):
+        b, p, n = _sz(), _sz(), ctypes.c_int()
+        check(lib().qcuServerUsage(self.handle, ctypes.byref(b), ctypes.byref(p), ctypes.byref(n)))
+        return {"bytes": b.value, "peak_bytes": p.value, "batches": n.value}
+
+    def destroy(self):
+        if self.handle:
+            check(lib().qcuServerDestroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

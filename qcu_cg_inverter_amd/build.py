@@ -27,7 +27,9 @@ SOURCES = [
     "k_gen.hip",
     "api.cpp",
     "solver.cpp",
+    "cg_batch.cpp",
     "tune.cpp",
+    "serve.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

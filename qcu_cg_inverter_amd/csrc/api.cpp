@@ -225,6 +225,18 @@ void set_tuned_hop(int prec, int recon, long vh, int nrhs, const Tune& t) {
 void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
          const void* clov, double coef, int nrhs, double* partials, hipStream_t st) {
   QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
+  const void* cl[kMaxRhs];
+  double co[kMaxRhs];
+  for (int r = 0; r < nrhs; ++r) {
+    cl[r] = clov;
+    co[r] = coef;
+  }
+  hop_v(g, prec, recon, epi, parity, dagger, tbc, in, out, x, cl, co, nrhs, partials, st);
+}
+
+void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
+           const void* const* clovs, const double* coefs, int nrhs, double* partials, hipStream_t st) {
+  QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
   HopArgs a;
   memset(&a, 0, sizeof a);
   a.in = in.p;
@@ -234,9 +246,11 @@ void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, do
   a.out_stride = out.stride;
   a.x_stride = x.stride;
   a.gauge = g.get(prec, recon);
+  a.clov_shared = 1;
   for (int r = 0; r < nrhs; ++r) {
-    a.clov[r] = clov;
-    a.coef[r] = coef;
+    a.clov[r] = clovs[r];
+    a.coef[r] = coefs[r];
+    if (clovs[r] != clovs[0]) a.clov_shared = 0;
   }
   a.partials = partials;
   a.geo = g.geo;
@@ -247,7 +261,6 @@ void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, do
   Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
   a.xcd_remap = t.xcd;
   a.walk = make_walk(g.geo, t.tz, t.tt);
-  a.clov_shared = 1;
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }
 
@@ -268,6 +281,7 @@ static GaugeField& dropin_gauge(void* gauge, const QcuParam* param, bool reload)
     auto g = make_gauge_from_aos(gauge, param->lattice_size);
     launch_gauge_reconstruct(g->aos.p, g->geo.vh, stream());
     QHIP(hipGetLastError());
+    if (it != c.dropin.end()) drop_cached_batches(it->second.get());
     c.dropin[gauge] = std::move(g);
   }
   return *c.dropin[gauge];
@@ -365,6 +379,7 @@ int qcuFinalize(void) {
     Context& c = ctx();
     if (!c.inited) return;
     QHIP(hipStreamSynchronize(c.stream));
+    drop_cached_batches(nullptr);
     c.dropin.clear();
     c.scratch.clear();
     QHIP(hipStreamDestroy(c.stream));
@@ -456,6 +471,7 @@ int qcuGaugeDestroy(qcuGauge h) {
   return guarded([&] {
     ensure_init();
     QHIP(hipStreamSynchronize(stream()));
+    drop_cached_batches(reinterpret_cast<GaugeField*>(h));
     delete reinterpret_cast<GaugeField*>(h);
   });
 }

@@ -1,0 +1,327 @@
+// cg_batch.cpp -- slot-based batched even-odd CGLS (algorithm in solver.cpp).
+#include "cg_batch.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace qcu {
+
+size_t CgBatch::bytes_for(long vh, int prec, int cap) {
+  size_t fb = spinor_bytes(F64, vh), pb = spinor_bytes(prec, vh), xb = spinor_bytes(F32, vh);
+  size_t s = 6 * fb * cap + (prec == F64 ? 4 : 5) * pb * cap + (prec == F64 ? 0 : xb * cap);
+  s += sizeof(double) * (4 * n_partials(vh) * cap + cap) + sizeof(CgScal) * cap + 2 * (size_t)vh * 192;
+  return s;
+}
+
+CgBatch::CgBatch(GaugeField& g, int prec, int recon, int t_boundary, bool clover, int capacity,
+                 const qcuSolveParams& sp, hipStream_t st)
+    : g_(g), prec_(prec), xprec_(prec == F64 ? F64 : F32), recon_(recon), tbc_(t_boundary), clover_(clover),
+      cap_(capacity), sp_(sp), st_(st) {
+  QREQ(capacity >= 1 && capacity <= kMaxRhs, "batch capacity must be 1..16");
+  QREQ(prec >= 0 && prec <= 2, "bad precision");
+  QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
+  QREQ(t_boundary == 1 || t_boundary == -1, "t_boundary must be +1 or -1");
+  if (sp_.chunk < 1) sp_.chunk = 1;
+  if (sp_.delta <= 0.0 || sp_.delta >= 1.0) sp_.delta = 0.1;
+  vh_ = g.geo.vh;
+  nw_ = n_partials(vh_);
+  fb_ = (long)spinor_bytes(F64, vh_);
+  pb_ = (long)spinor_bytes(prec, vh_);
+  xb_ = (long)spinor_bytes(xprec_, vh_);
+  f64_.alloc((size_t)6 * fb_ * cap_);
+  char* f = f64_.c();
+  be_ = vec(f, fb_);
+  bo_ = vec(f + 1 * fb_ * cap_, fb_);
+  bh_ = vec(f + 2 * fb_ * cap_, fb_);
+  xe_ = vec(f + 3 * fb_ * cap_, fb_);
+  shi_ = vec(f + 4 * fb_ * cap_, fb_);
+  t64_ = vec(f + 5 * fb_ * cap_, fb_);
+  int nslo = prec == F64 ? 4 : 5;
+  slo_.alloc((size_t)nslo * pb_ * cap_);
+  char* s = slo_.c();
+  R_ = vec(s, pb_);
+  P_ = vec(s + 1 * pb_ * cap_, pb_);
+  T_ = vec(s + 2 * pb_ * cap_, pb_);
+  TO_ = vec(s + 3 * pb_ * cap_, pb_);
+  S_ = prec == F64 ? shi_ : vec(s + 4 * pb_ * cap_, pb_);
+  if (prec == F64) {
+    X_ = xe_;
+  } else {
+    xlo_.alloc((size_t)xb_ * cap_);
+    X_ = vec(xlo_.p, xb_);
+  }
+  part_.alloc(sizeof(double) * (4 * nw_ * cap_ + cap_));
+  double* pp = static_cast<double*>(part_.p);
+  part_rr_ = pp;
+  part_tt_ = pp + nw_ * cap_;
+  part_ss_ = pp + 2 * nw_ * cap_;
+  sums_ = pp + 4 * nw_ * cap_;
+  scal_.alloc(sizeof(CgScal) * cap_);
+  dsc_ = static_cast<CgScal*>(scal_.p);
+  QHIP(hipHostMalloc((void**)&hsc_, sizeof(CgScal) * cap_, 0));
+  g.get(prec, recon);
+  g.get(F64, recon);
+  QHIP(hipStreamSynchronize(ctx().stream));  // packed fields were made on the context stream
+  const char* e = getenv("QCU_DEBUG");
+  debug_ = e && e[0] && e[0] != '0';
+}
+
+CgBatch::~CgBatch() {
+  clear_graphs();
+  if (hsc_) (void)hipHostFree(hsc_);
+}
+
+void CgBatch::clear_graphs() {
+  for (auto& gr : graphs_) {
+    if (gr.second) (void)hipGraphExecDestroy(gr.second);
+    if (gr.first) (void)hipGraphDestroy(gr.first);
+  }
+  graphs_.clear();
+}
+
+void CgBatch::check_operator(const SlotSpec& s) const {
+  QREQ(std::isfinite(s.kappa) && std::isfinite(s.csw), "kappa and csw must be finite");
+  QREQ((s.csw != 0.0) == clover_, "right-hand side does not match the batch's operator type");
+  QREQ(s.tol > 0.0 && std::isfinite(s.tol), "tol must be positive");
+}
+
+double CgBatch::norm2_slot(Vec v, int r) {
+  double* part = part_ss_ + nw_ * cap_;  // spare partials region
+  launch_norm2(F64, at(v, r), vh_, 1, part, st_);
+  launch_sum_partials(part, nw_, 1, sums_, st_);
+  double out = 0.0;
+  QHIP(hipMemcpyAsync(&out, sums_, sizeof(double), hipMemcpyDeviceToHost, st_));
+  QHIP(hipStreamSynchronize(st_));
+  return out;
+}
+
+// out = Mhat in (Mhat^dag if dag) for slots [r0, r0 + n), each with its own kappa and
+// clover; tmp holds the odd intermediate
+void CgBatch::mhat(int prec, int dag, Vec in, Vec out, Vec tmp, int r0, int n, double* partials) {
+  const void* c1[kMaxRhs];
+  const void* c2[kMaxRhs];
+  double k1[kMaxRhs], k2[kMaxRhs];
+  for (int j = 0; j < n; ++j) {
+    const Slot& sl = slots_[r0 + j];
+    c1[j] = sl.cf ? sl.cf->ainv(prec, 1) : nullptr;
+    c2[j] = sl.cf ? sl.cf->a(prec, 0) : nullptr;
+    k1[j] = 1.0;
+    k2[j] = -sl.spec.kappa * sl.spec.kappa;
+  }
+  hop_v(g_, prec, recon_, clover_ ? EPI_CLOV : EPI_PLAIN, 1, dag, (double)tbc_, at(in, r0), at(tmp, r0), vec(nullptr),
+        c1, k1, n, nullptr, st_);
+  hop_v(g_, prec, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 0, dag, (double)tbc_, at(tmp, r0), at(out, r0),
+        at(in, r0), c2, k2, n, partials ? partials + (long)r0 * nw_ : nullptr, st_);
+}
+
+void CgBatch::add(const SlotSpec& spec) {
+  QREQ(n_ < cap_, "batch is full");
+  check_operator(spec);
+  const int r = n_;
+  Slot sl;
+  sl.spec = spec;
+  sl.updates = 0;
+  sl.cf = nullptr;
+  if (clover_) {
+    sl.cf = g_.clover(spec.kappa * spec.csw, F64);
+    if (prec_ != F64) g_.clover(spec.kappa * spec.csw, prec_);
+    QHIP(hipStreamSynchronize(ctx().stream));
+  }
+  if (sl.spec.max_iter <= 0) sl.spec.max_iter = (int)std::min<long>(20 * vh_, 1000000);
+  const void* b = spec.b;
+  if (!b) {
+    aos_.ensure((size_t)2 * vh_ * 192);
+    launch_source_gauss(aos_.p, 2 * vh_ * 12, spec.seed, st_);
+    b = aos_.p;
+  }
+  launch_spinor_from_aos(F64, b, 0, at(be_, r), vh_, 1, st_);
+  launch_spinor_from_aos(F64, static_cast<const char*>(b) + vh_ * 192, 0, at(bo_, r), vh_, 1, st_);
+  QHIP(hipGetLastError());
+  slots_.push_back(sl);
+  n_++;
+  Slot& s = slots_[r];
+  s.bnorm2 = norm2_slot(be_, r) + norm2_slot(bo_, r);
+  if (!std::isfinite(s.bnorm2)) {
+    slots_.pop_back();
+    n_--;
+    throw Error(QCU_ERROR_NOT_FINITE, "source contains NaN or Inf");
+  }
+  // b^ = b_e + kappa D_eo A_oo^-1 b_o
+  const double kap = spec.kappa;
+  if (clover_) {
+    launch_clover_apply(F64, s.cf->ainv(F64, 1), at(bo_, r), at(t64_, r), vh_, 1, st_);
+    hop(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(t64_, r), at(bh_, r), at(be_, r), nullptr, kap, 1, nullptr,
+        st_);
+  } else {
+    hop(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(bo_, r), at(bh_, r), at(be_, r), nullptr, kap, 1, nullptr,
+        st_);
+  }
+  launch_zero(F64, at(xe_, r), vh_, 1, st_);
+  QHIP(hipMemcpyAsync(at(shi_, r).p, at(bh_, r).p, fb_, hipMemcpyDeviceToDevice, st_));
+  if (prec_ != F64) {
+    launch_spinor_convert(F64, at(shi_, r), prec_, at(S_, r), vh_, 1, 0, st_);
+    launch_zero(xprec_, at(X_, r), vh_, 1, st_);
+  }
+  launch_zero(prec_, at(P_, r), vh_, 1, st_);  // scratch may hold stale, non-finite bits
+  QHIP(hipGetLastError());
+  double ss0 = norm2_slot(shi_, r);
+  s.t0 = std::chrono::steady_clock::now();
+  CgScal& c = hsc_[r];
+  memset(&c, 0, sizeof c);
+  c.ss = ss0;
+  c.ss_ref = ss0;
+  c.target2 = spec.tol * spec.tol * s.bnorm2;
+  c.first = 1;
+  c.converged = ss0 <= c.target2;
+  c.active = 1;
+  clear_graphs();
+  if (c.converged) {
+    finish(r, false);
+  } else {
+    QHIP(hipMemcpyAsync(dsc_ + r, hsc_ + r, sizeof(CgScal), hipMemcpyHostToDevice, st_));
+  }
+}
+
+void CgBatch::launch_chunk() {
+  const int n = n_;
+  if (n == 0) return;
+  auto iteration = [&]() {
+    mhat(prec_, 1, S_, R_, TO_, 0, n, part_rr_);
+    launch_cg_beta(part_rr_, nw_, dsc_, n, st_);
+    launch_cg_xpay(prec_, R_, P_, dsc_, vh_, n, st_);
+    mhat(prec_, 0, P_, T_, TO_, 0, n, part_tt_);
+    launch_cg_alpha(part_tt_, nw_, dsc_, n, st_);
+    launch_cg_update(prec_, xprec_, X_, P_, T_, S_, dsc_, part_ss_, vh_, n, st_);
+    launch_cg_ss(part_ss_, nw_, dsc_, n, st_);
+  };
+  if (!sp_.graph) {
+    for (int i = 0; i < sp_.chunk; ++i) iteration();
+    QHIP(hipGetLastError());
+    return;
+  }
+  if ((int)graphs_.size() <= n) graphs_.resize(n + 1, {nullptr, nullptr});
+  auto& gr = graphs_[n];
+  if (!gr.second) {
+    QHIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < sp_.chunk; ++i) iteration();
+    QHIP(hipStreamEndCapture(st_, &gr.first));
+    QHIP(hipGraphInstantiate(&gr.second, gr.first, nullptr, nullptr, 0));
+  }
+  QHIP(hipGraphLaunch(gr.second, st_));
+}
+
+// s = b^ - Mhat x in FP64 for slot r (the low-precision iterate folded into x first)
+double CgBatch::reliable_update(int r) {
+  if (prec_ != F64) {
+    launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
+    launch_zero(xprec_, at(X_, r), vh_, 1, st_);
+  }
+  mhat(F64, 0, xe_, shi_, t64_, r, 1, nullptr);
+  launch_axpby(F64, 1.0, at(bh_, r), -1.0, at(shi_, r), vh_, 1, st_);
+  double ss = norm2_slot(shi_, r);
+  if (prec_ != F64) launch_spinor_convert(F64, at(shi_, r), prec_, at(S_, r), vh_, 1, 0, st_);
+  slots_[r].updates++;
+  if (debug_) fprintf(stderr, "qcu cg reliable update tag=%llu |s|^2=%.6e\n", (unsigned long long)slots_[r].spec.tag, ss);
+  return ss;
+}
+
+void CgBatch::harvest() {
+  const int n = n_;
+  if (n == 0) return;
+  QHIP(hipMemcpyAsync(hsc_, dsc_, sizeof(CgScal) * n, hipMemcpyDeviceToHost, st_));
+  QHIP(hipStreamSynchronize(st_));
+  if (debug_)
+    for (int r = 0; r < n; ++r)
+      fprintf(stderr, "qcu cg tag=%llu it=%d ss=%.6e gamma=%.6e tt=%.6e alpha=%.6e beta=%.6e conv=%d brk=%d\n",
+              (unsigned long long)slots_[r].spec.tag, hsc_[r].iter, hsc_[r].ss, hsc_[r].gamma, hsc_[r].tt,
+              hsc_[r].alpha, hsc_[r].beta, hsc_[r].converged, hsc_[r].breakdown);
+  // top down, so the slot moved into a finished one has already been looked at
+  for (int r = n - 1; r >= 0; --r) {
+    CgScal& s = hsc_[r];
+    if (s.breakdown || !std::isfinite(s.ss)) {
+      finish(r, true);
+      continue;
+    }
+    const bool cap = s.iter >= slots_[r].spec.max_iter;
+    const bool need = s.converged || cap || (prec_ != F64 && s.ss < sp_.delta * sp_.delta * s.ss_ref);
+    if (!need) continue;
+    double ss = reliable_update(r);
+    if (!std::isfinite(ss)) {
+      finish(r, true);
+      continue;
+    }
+    s.ss = ss;
+    s.ss_ref = ss;
+    s.converged = ss <= s.target2;
+    if (s.converged || cap) finish(r, false);
+  }
+  if (n_ > 0) QHIP(hipMemcpyAsync(dsc_, hsc_, sizeof(CgScal) * n_, hipMemcpyHostToDevice, st_));
+}
+
+// x_o, true residual, solution output and stats for slot r, then compaction
+void CgBatch::finish(int r, bool breakdown) {
+  Slot& sl = slots_[r];
+  const double kap = sl.spec.kappa;
+  if (prec_ != F64) {
+    launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
+    launch_zero(xprec_, at(X_, r), vh_, 1, st_);
+  }
+  Vec xo = at(shi_, r);
+  if (clover_) {
+    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), at(t64_, r), at(bo_, r), nullptr, kap, 1, nullptr,
+        st_);
+    launch_clover_apply(F64, sl.cf->ainv(F64, 1), at(t64_, r), xo, vh_, 1, st_);
+  } else {
+    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), xo, at(bo_, r), nullptr, kap, 1, nullptr, st_);
+  }
+  // r = b - M x with M = A - kappa D on both parities
+  Vec re = at(bh_, r), ro = at(t64_, r);
+  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 0, 0, (double)tbc_, xo, re, at(xe_, r),
+      clover_ ? sl.cf->a(F64, 0) : nullptr, -kap, 1, nullptr, st_);
+  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), ro, xo,
+      clover_ ? sl.cf->a(F64, 1) : nullptr, -kap, 1, nullptr, st_);
+  launch_axpby(F64, 1.0, at(be_, r), -1.0, re, vh_, 1, st_);
+  launch_axpby(F64, 1.0, at(bo_, r), -1.0, ro, vh_, 1, st_);
+  if (sl.spec.x) {
+    launch_spinor_to_aos(F64, at(xe_, r), sl.spec.x, 0, vh_, 1, st_);
+    launch_spinor_to_aos(F64, xo, static_cast<char*>(sl.spec.x) + vh_ * 192, 0, vh_, 1, st_);
+  }
+  QHIP(hipGetLastError());
+  double nr = norm2_slot(re, r) + norm2_slot(ro, r);
+  SlotResult res;
+  res.tag = sl.spec.tag;
+  res.stats.iterations = hsc_[r].iter;
+  res.stats.reliable_updates = sl.updates;
+  res.stats.breakdown = breakdown ? 1 : 0;
+  res.stats.converged = (!breakdown && hsc_[r].converged) ? 1 : 0;
+  res.stats.true_residual = sl.bnorm2 > 0 ? std::sqrt(nr / sl.bnorm2) : 0.0;
+  res.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
+  finished_.push_back(res);
+  const int last = n_ - 1;
+  if (r != last) move_slot(last, r);
+  slots_.pop_back();
+  n_--;
+  clear_graphs();
+}
+
+void CgBatch::move_slot(int from, int to) {
+  Vec keep[] = {be_, bo_, bh_, xe_, shi_, P_};
+  for (const Vec& v : keep)
+    QHIP(hipMemcpyAsync(at(v, to).p, at(v, from).p, v.stride, hipMemcpyDeviceToDevice, st_));
+  if (prec_ != F64) {
+    QHIP(hipMemcpyAsync(at(S_, to).p, at(S_, from).p, S_.stride, hipMemcpyDeviceToDevice, st_));
+    QHIP(hipMemcpyAsync(at(X_, to).p, at(X_, from).p, X_.stride, hipMemcpyDeviceToDevice, st_));
+  }
+  slots_[to] = slots_[from];
+  hsc_[to] = hsc_[from];
+}
+
+std::vector<SlotResult> CgBatch::take_finished() {
+  std::vector<SlotResult> out;
+  out.swap(finished_);
+  return out;
+}
+
+}  // namespace qcu

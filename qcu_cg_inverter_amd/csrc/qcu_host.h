@@ -111,11 +111,16 @@ void set_last_error(int code, const std::string& msg);
 // one hop launch on nrhs strided fields
 void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
          const void* clov, double coef, int nrhs, double* partials, hipStream_t st);
+// the same with a clover field and coefficient per right-hand side
+void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
+           const void* const* clovs, const double* coefs, int nrhs, double* partials, hipStream_t st);
 
 // solver entry (solver.cpp)
 void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& sp, int nrhs, void* const* x,
                  const void* const* b, qcuSolveStats* stats);
 void apply_mhat_iface(GaugeField& g, const qcuOperator& op, void* out, const void* in, int dagger, int prec);
+// forget solver state that refers to gauge g (all gauges if null)
+void drop_cached_batches(GaugeField* g);
 void apply_m_iface(GaugeField& g, const qcuOperator& op, void* out, const void* in, int dagger);
 
 }  // namespace qcu

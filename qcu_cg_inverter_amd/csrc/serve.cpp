@@ -1,0 +1,436 @@
+// serve.cpp -- request loop with continuous batching over resident gauge fields.
+//
+// The reference has no serving path; this is the caller side of its solver.  One
+// worker thread owns the device work.  Each scheduling round it
+//   1. admits queued right-hand sides into batches (one CgBatch per gauge and
+//      operator type, each on its own stream) as slots and the HBM budget allow,
+//   2. launches one chunk of CG iterations on every batch that has work, so batches
+//      on different gauges overlap on the device,
+//   3. harvests: reliable updates, converged right-hand sides leave their slots, and
+//      requests whose right-hand sides are all done complete.
+// Empty batches are released when queued work needs their memory.  A right-hand side
+// enters at a chunk boundary and every decision about it uses only its own numbers,
+// so its result is the same whichever requests share its batch.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+
+#include "cg_batch.h"
+#include "qcu_host.h"
+
+namespace qcu {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  qcuRequest spec;
+  std::vector<void*> x;
+  std::vector<const void*> b;
+  int admitted = 0, finished = 0;
+  bool all_converged = true;
+  int max_iter_seen = 0;
+  double worst_res = 0.0;
+  Clock::time_point submit, first_admit, done_at;
+  bool started = false, done = false;
+};
+
+struct BatchEntry {
+  std::unique_ptr<CgBatch> batch;
+  hipStream_t stream = nullptr;
+  GaugeField* gauge = nullptr;
+  bool clover = false;
+  int recon = 0, tbc = 0;
+  size_t bytes = 0;
+  std::map<uint64_t, std::pair<uint64_t, int>> slot_req;  // slot tag -> (request id, rhs)
+};
+
+struct Server {
+  qcuServerConfig cfg;
+  size_t budget = 0, used = 0, peak = 0;
+  std::thread worker;
+  std::mutex mu;  // guards queue, requests, stop
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  std::deque<uint64_t> queue;  // requests with right-hand sides not yet admitted
+  std::map<uint64_t, Request> reqs;
+  uint64_t next_id = 1, next_tag = 1;
+  std::vector<std::unique_ptr<BatchEntry>> batches;
+  std::map<std::pair<GaugeField*, double>, int> clover_refs;  // (gauge, kappa*csw) -> slots using it
+  std::string error;
+
+  void run();
+  bool admit_one(uint64_t id, Request& r);
+  BatchEntry* find_batch(GaugeField* g, bool clover, int recon, int tbc);
+  BatchEntry* new_batch(GaugeField* g, bool clover, int recon, int tbc);
+  size_t clover_cost(GaugeField* g, double ck);
+  void release_clover(GaugeField* g, double ck);
+  void round();
+};
+
+size_t Server::clover_cost(GaugeField* g, double ck) {
+  if (clover_refs.count({g, ck})) return 0;
+  size_t per = clover_bytes(F64, g->geo.vh) * 2;
+  if (cfg.precision != F64) per += clover_bytes(cfg.precision, g->geo.vh) * 2;
+  return per;
+}
+
+void Server::release_clover(GaugeField* g, double ck) {
+  auto it = clover_refs.find({g, ck});
+  if (it == clover_refs.end()) return;
+  if (--it->second > 0) return;
+  clover_refs.erase(it);
+  size_t per = clover_bytes(F64, g->geo.vh) * 2;
+  if (cfg.precision != F64) per += clover_bytes(cfg.precision, g->geo.vh) * 2;
+  auto c = g->clovers.find(ck);
+  if (c != g->clovers.end()) {
+    QHIP(hipStreamSynchronize(ctx().stream));
+    g->clovers.erase(c);
+  }
+  used -= per;
+}
+
+BatchEntry* Server::find_batch(GaugeField* g, bool clover, int recon, int tbc) {
+  for (auto& b : batches)
+    if (b->gauge == g && b->clover == clover && b->recon == recon && b->tbc == tbc && b->batch->free_slots() > 0)
+      return b.get();
+  return nullptr;
+}
+
+BatchEntry* Server::new_batch(GaugeField* g, bool clover, int recon, int tbc) {
+  size_t need = CgBatch::bytes_for(g->geo.vh, cfg.precision, cfg.max_batch);
+  if (used + need > budget) {
+    // give back idle batches first
+    for (size_t i = 0; i < batches.size();) {
+      if (batches[i]->batch->active() == 0) {
+        used -= batches[i]->bytes;
+        QHIP(hipStreamSynchronize(batches[i]->stream));
+        QHIP(hipStreamDestroy(batches[i]->stream));
+        batches.erase(batches.begin() + i);
+      } else {
+        ++i;
+      }
+    }
+    if (used + need > budget) return nullptr;
+  }
+  std::unique_ptr<BatchEntry> e(new BatchEntry);
+  QHIP(hipStreamCreate(&e->stream));  // default priority
+  qcuSolveParams sp;
+  qcuDefaultSolveParams(&sp);
+  sp.precision = cfg.precision;
+  sp.chunk = cfg.chunk;
+  sp.graph = cfg.graph;
+  e->batch.reset(new CgBatch(*g, cfg.precision, recon, tbc, clover, cfg.max_batch, sp, e->stream));
+  e->gauge = g;
+  e->clover = clover;
+  e->recon = recon;
+  e->tbc = tbc;
+  e->bytes = need;
+  used += need;
+  if (used > peak) peak = used;
+  batches.push_back(std::move(e));
+  return batches.back().get();
+}
+
+// move as many right-hand sides of request id into batches as fit; true when all are in
+bool Server::admit_one(uint64_t id, Request& r) {
+  GaugeField* g = reinterpret_cast<GaugeField*>(r.spec.gauge);
+  const bool clover = r.spec.csw != 0.0;
+  const double ck = r.spec.kappa * r.spec.csw;
+  while (r.admitted < r.spec.nrhs) {
+    BatchEntry* be = find_batch(g, clover, r.spec.recon, r.spec.t_boundary);
+    if (clover) {
+      size_t cc = clover_cost(g, ck);
+      if (cc && used + cc > budget) return false;
+    }
+    if (!be) be = new_batch(g, clover, r.spec.recon, r.spec.t_boundary);
+    if (!be) return false;
+    if (clover) {
+      size_t cc = clover_cost(g, ck);
+      used += cc;
+      if (used > peak) peak = used;
+      clover_refs[{g, ck}]++;
+    }
+    const int k = r.admitted;
+    SlotSpec s;
+    s.b = r.b.empty() ? nullptr : r.b[k];
+    s.seed = r.spec.seed + (uint64_t)k;
+    s.x = r.x.empty() ? nullptr : r.x[k];
+    s.kappa = r.spec.kappa;
+    s.csw = r.spec.csw;
+    s.tol = r.spec.tol;
+    s.max_iter = r.spec.max_iter;
+    s.tag = next_tag++;
+    be->slot_req[s.tag] = {id, k};
+    be->batch->add(s);  // device work: the request lock is not held here
+    std::lock_guard<std::mutex> lk(mu);
+    if (!r.started) {
+      r.started = true;
+      r.first_admit = Clock::now();
+    }
+    r.admitted++;
+  }
+  return true;
+}
+
+void Server::round() {
+  // 1. admission, in arrival order; a request that does not fit keeps its place
+  std::vector<uint64_t> ids;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    ids.assign(queue.begin(), queue.end());
+  }
+  std::vector<uint64_t> admitted_all;
+  for (uint64_t id : ids) {
+    Request* r;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      r = &reqs[id];
+    }
+    if (admit_one(id, *r)) admitted_all.push_back(id);
+    else break;  // keep arrival order for the memory budget
+  }
+  if (!admitted_all.empty()) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (uint64_t id : admitted_all)
+      for (auto it = queue.begin(); it != queue.end(); ++it)
+        if (*it == id) {
+          queue.erase(it);
+          break;
+        }
+  }
+  // 2. one chunk on every busy batch, all in flight together
+  for (auto& b : batches)
+    if (b->batch->active() > 0) b->batch->launch_chunk();
+  // 3. harvest
+  for (auto& b : batches) {
+    if (b->batch->active() == 0) continue;
+    b->batch->harvest();
+    for (const SlotResult& res : b->batch->take_finished()) {
+      auto it = b->slot_req.find(res.tag);
+      if (it == b->slot_req.end()) continue;
+      uint64_t id = it->second.first;
+      b->slot_req.erase(it);
+      std::lock_guard<std::mutex> lk(mu);
+      Request& r = reqs[id];
+      r.finished++;
+      r.all_converged &= res.stats.converged != 0;
+      r.max_iter_seen = std::max(r.max_iter_seen, res.stats.iterations);
+      r.worst_res = std::max(r.worst_res, res.stats.true_residual);
+      if (r.spec.csw != 0.0) release_clover(b->gauge, r.spec.kappa * r.spec.csw);
+      if (r.finished == r.spec.nrhs) {
+        r.done = true;
+        r.done_at = Clock::now();
+        cv_done.notify_all();
+      }
+    }
+  }
+}
+
+void Server::run() {
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      bool busy = false;
+      for (auto& b : batches) busy |= b->batch->active() > 0;
+      if (!busy) cv_work.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !queue.empty(); });
+      if (stop) break;
+    }
+    try {
+      std::lock_guard<std::recursive_mutex> g(ctx().mu);
+      round();
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> lk(mu);
+      error = e.what();
+      // fail every pending request rather than hang their waiters
+      for (auto& kv : reqs)
+        if (!kv.second.done) {
+          kv.second.done = true;
+          kv.second.all_converged = false;
+          kv.second.done_at = Clock::now();
+        }
+      queue.clear();
+      cv_done.notify_all();
+      stop = true;
+      break;
+    }
+  }
+}
+
+static void fill_status(const Request& r, qcuRequestStatus* st) {
+  if (!st) return;
+  st->done = r.done;
+  st->converged = r.done && r.all_converged;
+  st->iterations = r.max_iter_seen;
+  st->true_residual = r.worst_res;
+  st->queue_seconds = r.started ? std::chrono::duration<double>(r.first_admit - r.submit).count() : -1.0;
+  st->latency_seconds = r.done ? std::chrono::duration<double>(r.done_at - r.submit).count() : -1.0;
+}
+
+}  // namespace qcu
+
+using namespace qcu;
+
+extern "C" {
+
+void qcuDefaultServerConfig(qcuServerConfig* cfg) {
+  if (!cfg) return;
+  cfg->hbm_budget = 0;
+  cfg->max_batch = 12;
+  cfg->precision = QCU_PREC_SINGLE;
+  cfg->chunk = 8;
+  cfg->graph = 1;
+}
+
+int qcuServerCreate(qcuServer* out, const qcuServerConfig* cfg) {
+  try {
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    ensure_init();
+    QREQ(out, "qcuServerCreate: NULL");
+    std::unique_ptr<Server> s(new Server);
+    if (cfg) s->cfg = *cfg;
+    else qcuDefaultServerConfig(&s->cfg);
+    QREQ(s->cfg.max_batch >= 1 && s->cfg.max_batch <= kMaxRhs, "max_batch must be 1..16");
+    QREQ(s->cfg.precision >= 0 && s->cfg.precision <= 2, "bad precision");
+    if (s->cfg.chunk < 1) s->cfg.chunk = 1;
+    s->budget = s->cfg.hbm_budget;
+    if (s->budget == 0) {
+      size_t fr = 0, tot = 0;
+      QHIP(hipMemGetInfo(&fr, &tot));
+      s->budget = fr / 4 * 3;
+    }
+    Server* raw = s.get();
+    s->worker = std::thread([raw] { raw->run(); });
+    *out = reinterpret_cast<qcuServer>(s.release());
+    return QCU_SUCCESS;
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    return e.code;
+  }
+}
+
+int qcuServerSubmit(qcuServer h, const qcuRequest* req, uint64_t* id) {
+  Server* s = reinterpret_cast<Server*>(h);
+  if (!s || !req || !id || !req->gauge) {
+    set_last_error(QCU_ERROR_INVALID_VALUE, "qcuServerSubmit: NULL argument");
+    return QCU_ERROR_INVALID_VALUE;
+  }
+  if (req->nrhs < 1 || req->nrhs > kMaxRhs || !(req->tol > 0.0) || (req->recon != 12 && req->recon != 18) ||
+      (req->t_boundary != 1 && req->t_boundary != -1)) {
+    set_last_error(QCU_ERROR_INVALID_VALUE, "qcuServerSubmit: bad request parameters");
+    return QCU_ERROR_INVALID_VALUE;
+  }
+  GaugeField* g = reinterpret_cast<GaugeField*>(req->gauge);
+  size_t need = CgBatch::bytes_for(g->geo.vh, s->cfg.precision, s->cfg.max_batch);
+  if (need > s->budget) {
+    set_last_error(QCU_ERROR_ADMISSION, "request needs more device memory than the server's budget");
+    return QCU_ERROR_ADMISSION;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) {
+    set_last_error(QCU_ERROR_STATE, "server stopped: " + s->error);
+    return QCU_ERROR_STATE;
+  }
+  uint64_t i = s->next_id++;
+  Request& r = s->reqs[i];
+  r.spec = *req;
+  if (req->x) r.x.assign(req->x, req->x + req->nrhs);
+  if (req->b) r.b.assign(req->b, req->b + req->nrhs);
+  r.spec.x = nullptr;
+  r.spec.b = nullptr;
+  r.submit = Clock::now();
+  s->queue.push_back(i);
+  s->cv_work.notify_one();
+  *id = i;
+  return QCU_SUCCESS;
+}
+
+int qcuServerPoll(qcuServer h, uint64_t id, qcuRequestStatus* st) {
+  Server* s = reinterpret_cast<Server*>(h);
+  if (!s) return QCU_ERROR_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) {
+    set_last_error(QCU_ERROR_INVALID_VALUE, "unknown request id");
+    return QCU_ERROR_INVALID_VALUE;
+  }
+  fill_status(it->second, st);
+  return QCU_SUCCESS;
+}
+
+int qcuServerWait(qcuServer h, uint64_t id, qcuRequestStatus* st) {
+  Server* s = reinterpret_cast<Server*>(h);
+  if (!s) return QCU_ERROR_INVALID_VALUE;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) {
+    set_last_error(QCU_ERROR_INVALID_VALUE, "unknown request id");
+    return QCU_ERROR_INVALID_VALUE;
+  }
+  s->cv_done.wait(lk, [&] { return it->second.done; });
+  fill_status(it->second, st);
+  if (!s->error.empty()) {
+    set_last_error(QCU_ERROR_STATE, "server failed: " + s->error);
+    return QCU_ERROR_STATE;
+  }
+  return it->second.all_converged ? QCU_SUCCESS : QCU_ERROR_NOT_CONVERGED;
+}
+
+int qcuServerDrain(qcuServer h) {
+  Server* s = reinterpret_cast<Server*>(h);
+  if (!s) return QCU_ERROR_INVALID_VALUE;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] {
+    for (auto& kv : s->reqs)
+      if (!kv.second.done) return false;
+    return true;
+  });
+  if (!s->error.empty()) {
+    set_last_error(QCU_ERROR_STATE, "server failed: " + s->error);
+    return QCU_ERROR_STATE;
+  }
+  return QCU_SUCCESS;
+}
+
+int qcuServerUsage(qcuServer h, size_t* bytes, size_t* peak, int* nb) {
+  Server* s = reinterpret_cast<Server*>(h);
+  if (!s) return QCU_ERROR_INVALID_VALUE;
+  std::lock_guard<std::recursive_mutex> g(ctx().mu);
+  if (bytes) *bytes = s->used;
+  if (peak) *peak = s->peak;
+  if (nb) *nb = (int)s->batches.size();
+  return QCU_SUCCESS;
+}
+
+int qcuServerDestroy(qcuServer h) {
+  Server* s = reinterpret_cast<Server*>(h);
+  if (!s) return QCU_ERROR_INVALID_VALUE;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  try {
+    std::lock_guard<std::recursive_mutex> g(ctx().mu);
+    for (auto& b : s->batches) {
+      QHIP(hipStreamSynchronize(b->stream));
+      b->batch.reset();
+      QHIP(hipStreamDestroy(b->stream));
+    }
+    for (auto& kv : s->clover_refs) {
+      auto c = kv.first.first->clovers.find(kv.first.second);
+      if (c != kv.first.first->clovers.end()) kv.first.first->clovers.erase(c);
+    }
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    delete s;
+    return e.code;
+  }
+  delete s;
+  return QCU_SUCCESS;
+}
+
+}  // extern "C"

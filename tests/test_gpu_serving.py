@@ -1,0 +1,105 @@
+"""GPU tests of the continuous-batching server (qcuServer*): every request converges
+to its tolerance (true residual recomputed in FP64 and checked by the oracle), a
+request gives the same bits alone and in a mix of other requests that arrive while
+it runs, and admission rejects what cannot fit the HBM budget."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def q():
+    import qcu_cg_inverter_amd as q
+
+    q.init(0)
+    return q
+
+
+def run_request(q, server, g, L, kappa, csw, tol, bs):
+    vh = q.half_volume(L)
+    xs = [q.DeviceBuffer(2 * vh * 192) for _ in bs]
+    rid = server.submit(g, kappa, csw, tol, len(bs), xs=xs, bs=bs)
+    return rid, xs
+
+
+def test_server_alone_equals_mix(q):
+    La, Lb = (8, 8, 8, 8), (8, 8, 4, 8)
+    ga = q.Gauge.random(La, seed=41, eps=0.25)
+    gb = q.Gauge.random(Lb, seed=42, eps=0.25)
+    vh = q.half_volume(La)
+    src = [q.DeviceBuffer.from_numpy(orc.source_gauss(La, 70 + i)) for i in range(3)]
+    # alone
+    s1 = q.Server(precision=1, max_batch=8, chunk=4)
+    rid, xs = run_request(q, s1, ga, La, 0.121, 1.0, 1e-10, src)
+    st = s1.wait(rid)
+    alone = [x.to_numpy((2, vh, 4, 3)) for x in xs]
+    it_alone = st.iterations
+    s1.destroy()
+    assert st.converged and st.true_residual <= 1e-10
+    # in a mix: others before, during and after, on both gauges, Wilson and clover,
+    # different kappa, some arriving while the request runs
+    s2 = q.Server(precision=1, max_batch=8, chunk=4)
+    others = []
+    for k in range(3):
+        others.append(s2.submit(ga, 0.115 + 0.002 * k, 1.0, 1e-8, 2, seed=100 + k))
+        others.append(s2.submit(gb, 0.12, 0.0, 1e-10, 1 + k, seed=200 + k))
+    time.sleep(0.002)
+    rid, xs = run_request(q, s2, ga, La, 0.121, 1.0, 1e-10, src)
+    for k in range(4):
+        time.sleep(0.001)
+        others.append(s2.submit(ga, 0.118, 0.0, 1e-10, 3, seed=300 + k))
+        others.append(s2.submit(ga, 0.121, 1.0, 1e-10, 2, seed=400 + k))
+    st2 = s2.wait(rid)
+    mix = [x.to_numpy((2, vh, 4, 3)) for x in xs]
+    for o in others:
+        so = s2.wait(o)
+        assert so.converged
+    s2.destroy()
+    assert st2.iterations == it_alone
+    for a, m in zip(alone, mix):
+        assert np.array_equal(a, m)
+    # and the answer is right (oracle residual in FP64)
+    gref = ga.to_numpy()
+    A = orc.clover_fields(gref, La, 0.121, 1.0, recon=12)[0]
+    for i in range(3):
+        b = src[i].to_numpy((2, vh, 4, 3))
+        r = b - orc.full_op(mix[i], gref, La, 0.121, A=A, s=1, tbc=-1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-10
+
+
+def test_server_burst_all_converge(q):
+    rng = np.random.default_rng(5)
+    Ls = [(8, 8, 8, 8), (16, 8, 8, 8)]
+    gs = [q.Gauge.random(L, seed=50 + i, eps=0.25) for i, L in enumerate(Ls)]
+    s = q.Server(precision=1, max_batch=12, chunk=8)
+    ids = []
+    for k in range(24):
+        gi = int(rng.integers(2))
+        kappa = float(rng.uniform(0.11, 0.124))
+        csw = 1.0 if rng.random() < 0.5 else 0.0
+        tol = 1e-8 if rng.random() < 0.5 else 1e-10
+        nrhs = int(rng.integers(1, 13))
+        ids.append((s.submit(gs[gi], kappa, csw, tol, nrhs, seed=1000 + 13 * k), tol))
+    s.drain()
+    for rid, tol in ids:
+        st = s.poll(rid)
+        assert st.done and st.converged and st.true_residual <= tol, st.as_dict()
+        assert st.latency_seconds >= st.queue_seconds >= 0
+    use = s.usage()
+    assert use["peak_bytes"] > 0
+    s.destroy()
+
+
+def test_server_rejects_over_budget(q):
+    L = (16, 16, 16, 16)
+    g = q.Gauge.random(L, seed=3, eps=0.2)
+    s = q.Server(precision=1, max_batch=12, hbm_budget=1 << 20)
+    with pytest.raises(q.QcuError) as e:
+        s.submit(g, 0.12, 0.0, 1e-8, 1, seed=1)
+    assert e.value.code == 8
+    s.destroy()
