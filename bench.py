@@ -281,6 +281,24 @@ def cpu_baseline(seconds_target=15.0):
             break
     vh = orc.half_volume(L)
     gbs = BYTES_PER_SITE[("f64", 18)] * vh * n / el / 1e9
+    # the same on one core, and on the 24^3x48 lattice with all of them
+    orc.set_threads(1)
+    n1, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 3.0:
+        orc.hop(v[1 - (n1 & 1)], g, L, n1 & 1, 1, -1.0, 18)
+        n1 += 1
+    el1 = time.perf_counter() - t0
+    orc.set_threads(threads)
+    L3 = (24, 24, 24, 48)
+    g3 = orc.gauge_random(L3, 1, 0.25)
+    v3 = orc.source_gauss(L3, 2)
+    orc.hop(v3[1], g3, L3, 0, 1, -1.0, 18)
+    n3, t0 = 0, time.perf_counter()
+    while n3 < 2 or time.perf_counter() - t0 < 3.0:
+        orc.hop(v3[1 - (n3 & 1)], g3, L3, n3 & 1, 1, -1.0, 18)
+        n3 += 1
+    el3 = time.perf_counter() - t0
+    del g3, v3
     # the CPU-runnable configuration: 8^4 Wilson, kappa 0.12, eps 0.1, antiperiodic t,
     # one Gaussian source, FP64 even-odd CG to 1e-10
     L1 = (8, 8, 8, 8)
@@ -293,6 +311,11 @@ def cpu_baseline(seconds_target=15.0):
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{n} FP64 hop applications on 16^3x32 ({el:.1f} s, oracle, OpenMP)",
             "ms_per_apply": round(1e3 * el / n, 3),
+            "one_core": {"GBps": round(BYTES_PER_SITE[("f64", 18)] * vh * n1 / el1 / 1e9, 3),
+                         "ms_per_apply": round(1e3 * el1 / n1, 3), "applications": n1},
+            "hop_24x24x24x48": {"GBps": round(BYTES_PER_SITE[("f64", 18)] * orc.half_volume(L3) * n3 / el3 / 1e9, 3),
+                                "ms_per_apply": round(1e3 * el3 / n3, 3), "applications": n3, "cores": threads},
+            "host_cpu": platform.processor() or platform.machine(),
             "cg_config1_8x8x8x8": {"seconds": round(tcg, 4), "iterations": st.iterations,
                                    "true_residual": st.true_residual}}
 
@@ -325,6 +348,8 @@ def main():
                     help="requests in the Poisson serving stream (2000 for the full configuration; 0 skips)")
     ap.add_argument("--serving-baseline", type=int, default=12,
                     help="requests of the stream re-run as plain FP64 one-at-a-time solves")
+    ap.add_argument("--kernels-12rhs", action="store_true",
+                    help="profiler driver: copy kernel, then the 32^3x64 FP32 hop with 1 and 12 right-hand sides")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiler driver: 1 GiB copy kernel x5 then the 16^3x32 FP64 hop x steps, nothing else")
     args = ap.parse_args()
@@ -343,6 +368,31 @@ def main():
         gk = q.Gauge.random(L_MAIN, seed=1, eps=0.2)
         ms = q.bench_hop(gk, precision=0, recon=18, nrhs=1, iters=args.steps)
         log(f"kernels-only: hop f64 r18 16^3x32 {ms*1e3:.2f} us/launch over {args.steps}")
+        return 0
+    if args.kernels_12rhs:
+        # profiler driver: 1 GiB copy x5, then the 32^3x64 FP32 recon-12 hop with 1
+        # right-hand side, with 12 in the autotuned shape, and with 12 in that shape
+        # with the other right-hand side group order.  With QCU_TUNE_CACHE naming an
+        # existing file the shapes come from it, so profiled runs launch no tuning.
+        q.bench_copy(1 << 30, iters=5)
+        gk = q.Gauge.random((32, 32, 32, 64), seed=3, eps=0.25)
+        cache = os.environ.get("QCU_TUNE_CACHE", "")
+        have = bool(cache) and os.path.exists(cache)
+        for n in (1, 12):
+            if not have:
+                q.tune_hop(gk, 1, 12, n)
+            ms = q.bench_hop(gk, precision=1, recon=12, nrhs=n, iters=args.steps)
+            log(f"kernels-12rhs: f32 r12 32^3x64 nrhs={n} {ms*1e3:.1f} us/launch")
+        if have:
+            key = f"hop:1:12:{q.half_volume(gk.L)}:12"
+            with open(cache) as f:
+                rows = [ln.split() for ln in f if ln.startswith(key + " ")]
+            sh = dict(zip(q.SHAPE_KEYS, map(int, rows[-1][1:8])))
+            if sh["rb"] * sh["rpt"] < 12:
+                sh["group_major"] ^= 1
+                q.set_hop_shape(gk, 1, 12, 12, sh)
+                ms = q.bench_hop(gk, precision=1, recon=12, nrhs=12, iters=args.steps)
+                log(f"kernels-12rhs: f32 r12 32^3x64 nrhs=12 other group order {sh} {ms*1e3:.1f} us/launch")
         return 0
     out = {"metric": METRIC, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",

@@ -134,6 +134,7 @@ PROTOTYPES = {
     "qcuBenchHop": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dbl)]),
     "qcuBenchCopy": (_int, [_sz, _int, ctypes.POINTER(_dbl)]),
     "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, ctypes.POINTER(_dbl)]),
+    "qcuSetHopShape": (_int, [_vp, _int, _int, _int, _i4]),
     "qcuDefaultServerConfig": (None, [ctypes.POINTER(ServerConfig)]),
     "qcuServerCreate": (_int, [ctypes.POINTER(_vp), ctypes.POINTER(ServerConfig)]),
     "qcuServerSubmit": (_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
@@ -386,10 +387,19 @@ def solve_batch_device(g: Gauge, op: Operator, xs, bs, params: SolveParams | Non
 
 def tune_hop(g: Gauge, precision=PREC_DOUBLE, recon=18, nrhs=1):
     """Autotune the hopping kernel's launch shape; returns (shape dict, ms)."""
-    v = (ctypes.c_int * 6)()
+    v = (ctypes.c_int * 7)()
     ms = ctypes.c_double()
     check(lib().qcuTuneHop(g.handle, precision, recon, nrhs, v, ctypes.byref(ms)))
-    return dict(zip(("block", "rb", "rpt", "xcd", "tile_z", "tile_t"), list(v))), ms.value
+    return dict(zip(SHAPE_KEYS, list(v))), ms.value
+
+
+SHAPE_KEYS = ("block", "rb", "rpt", "xcd", "tile_z", "tile_t", "group_major")
+
+
+def set_hop_shape(g: Gauge, precision, recon, nrhs, shape: dict):
+    """Pin the hopping kernel's launch shape for (precision, recon, nrhs) on g's volume."""
+    v = (ctypes.c_int * 7)(*[int(shape.get(k, 0)) for k in SHAPE_KEYS])
+    check(lib().qcuSetHopShape(g.handle, precision, recon, nrhs, v))
 
 
 def bench_copy(nbytes, iters=20) -> float:

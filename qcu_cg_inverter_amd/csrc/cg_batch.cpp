@@ -87,6 +87,14 @@ void CgBatch::check_operator(const SlotSpec& s) const {
   QREQ(s.tol > 0.0 && std::isfinite(s.tol), "tol must be positive");
 }
 
+Vec CgBatch::at(Vec v, int r) const {
+  const Vec base[] = {be_, bo_, bh_, xe_, shi_, t64_, S_, R_, P_, T_, TO_, X_};
+  bool ok = false;
+  for (const Vec& b : base) ok |= b.p == v.p && b.stride == v.stride;
+  if (!ok || r < 0 || r >= cap_) throw Error(QCU_ERROR_STATE, "internal: batch slot view out of range");
+  return vec(static_cast<char*>(v.p) + (long)r * v.stride, v.stride);
+}
+
 double CgBatch::norm2_slot(Vec v, int r) {
   double* part = part_ss_ + nw_ * cap_;  // spare partials region
   launch_norm2(F64, at(v, r), vh_, 1, part, st_);
@@ -100,6 +108,7 @@ double CgBatch::norm2_slot(Vec v, int r) {
 // out = Mhat in (Mhat^dag if dag) for slots [r0, r0 + n), each with its own kappa and
 // clover; tmp holds the odd intermediate
 void CgBatch::mhat(int prec, int dag, Vec in, Vec out, Vec tmp, int r0, int n, double* partials) {
+  if (r0 < 0 || n < 1 || r0 + n > n_) throw Error(QCU_ERROR_STATE, "internal: batch slot range out of range");
   const void* c1[kMaxRhs];
   const void* c2[kMaxRhs];
   double k1[kMaxRhs], k2[kMaxRhs];
@@ -237,27 +246,76 @@ void CgBatch::harvest() {
       fprintf(stderr, "qcu cg tag=%llu it=%d ss=%.6e gamma=%.6e tt=%.6e alpha=%.6e beta=%.6e conv=%d brk=%d\n",
               (unsigned long long)slots_[r].spec.tag, hsc_[r].iter, hsc_[r].ss, hsc_[r].gamma, hsc_[r].tt,
               hsc_[r].alpha, hsc_[r].beta, hsc_[r].converged, hsc_[r].breakdown);
-  // top down, so the slot moved into a finished one has already been looked at
-  for (int r = n - 1; r >= 0; --r) {
-    CgScal& s = hsc_[r];
+  std::vector<char> brk(n, 0), upd(n, 0), cap(n, 0);
+  int lo = n, hi = -1;
+  for (int r = 0; r < n; ++r) {
+    const CgScal& s = hsc_[r];
     if (s.breakdown || !std::isfinite(s.ss)) {
-      finish(r, true);
+      brk[r] = 1;
       continue;
     }
-    const bool cap = s.iter >= slots_[r].spec.max_iter;
-    const bool need = s.converged || cap || (prec_ != F64 && s.ss < sp_.delta * sp_.delta * s.ss_ref);
-    if (!need) continue;
-    double ss = reliable_update(r);
-    if (!std::isfinite(ss)) {
-      finish(r, true);
-      continue;
+    cap[r] = s.iter >= slots_[r].spec.max_iter;
+    if (s.converged || cap[r] || (prec_ != F64 && s.ss < sp_.delta * sp_.delta * s.ss_ref)) {
+      upd[r] = 1;
+      lo = std::min(lo, r);
+      hi = std::max(hi, r);
     }
-    s.ss = ss;
-    s.ss_ref = ss;
-    s.converged = ss <= s.target2;
-    if (s.converged || cap) finish(r, false);
+  }
+  if (hi >= lo) {
+    std::vector<double> ss = reliable_updates(lo, hi, upd);
+    for (int r = lo; r <= hi; ++r) {
+      if (!upd[r]) continue;
+      CgScal& s = hsc_[r];
+      s.ss = ss[r - lo];
+      s.ss_ref = s.ss;
+      s.converged = s.ss <= s.target2;
+      if (!std::isfinite(s.ss)) brk[r] = 1;
+    }
+  }
+  // top down, so the slot moved into a finished one has already been dealt with
+  for (int r = n - 1; r >= 0; --r) {
+    if (brk[r]) finish(r, true);
+    else if (upd[r] && (hsc_[r].converged || cap[r])) finish(r, false);
   }
   if (n_ > 0) QHIP(hipMemcpyAsync(dsc_, hsc_, sizeof(CgScal) * n_, hipMemcpyHostToDevice, st_));
+}
+
+// s = b^ - Mhat x in FP64 for the slots r in [lo, hi] with upd[r] set, the
+// low-precision iterate of each folded into x first; returns |s|^2 per slot of the
+// range.  With a low-precision loop the FP64 residual fields are scratch between
+// updates, so the operator runs once over the whole range (the per-slot arithmetic
+// is the same as alone); an FP64 loop updates its live residual slot by slot.
+std::vector<double> CgBatch::reliable_updates(int lo, int hi, const std::vector<char>& upd) {
+  const int n = hi - lo + 1;
+  std::vector<double> out(n, 0.0);
+  if (prec_ == F64) {
+    for (int r = lo; r <= hi; ++r)
+      if (upd[r]) out[r - lo] = reliable_update(r);
+    return out;
+  }
+  for (int r = lo; r <= hi; ++r) {
+    if (!upd[r]) continue;
+    launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
+    launch_zero(xprec_, at(X_, r), vh_, 1, st_);
+  }
+  mhat(F64, 0, xe_, shi_, t64_, lo, n, nullptr);
+  launch_axpby(F64, 1.0, at(bh_, lo), -1.0, at(shi_, lo), vh_, n, st_);
+  double* part = part_ss_ + nw_ * cap_;
+  launch_norm2(F64, at(shi_, lo), vh_, n, part, st_);
+  launch_sum_partials(part, nw_, n, sums_, st_);
+  QHIP(hipMemcpyAsync(out.data(), sums_, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
+  for (int r = lo; r <= hi; ++r) {
+    if (!upd[r]) continue;
+    launch_spinor_convert(F64, at(shi_, r), prec_, at(S_, r), vh_, 1, 0, st_);
+    slots_[r].updates++;
+  }
+  QHIP(hipStreamSynchronize(st_));
+  if (debug_)
+    for (int r = lo; r <= hi; ++r)
+      if (upd[r])
+        fprintf(stderr, "qcu cg reliable update tag=%llu |s|^2=%.6e\n", (unsigned long long)slots_[r].spec.tag,
+                out[r - lo]);
+  return out;
 }
 
 // x_o, true residual, solution output and stats for slot r, then compaction
@@ -289,7 +347,7 @@ void CgBatch::finish(int r, bool breakdown) {
     launch_spinor_to_aos(F64, xo, static_cast<char*>(sl.spec.x) + vh_ * 192, 0, vh_, 1, st_);
   }
   QHIP(hipGetLastError());
-  double nr = norm2_slot(re, r) + norm2_slot(ro, r);
+  double nr = norm2_slot(bh_, r) + norm2_slot(t64_, r);  // re, ro
   SlotResult res;
   res.tag = sl.spec.tag;
   res.stats.iterations = hsc_[r].iter;

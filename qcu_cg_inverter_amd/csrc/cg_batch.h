@@ -65,9 +65,11 @@ class CgBatch {
   void check_operator(const SlotSpec& s) const;
   void mhat(int prec, int dag, Vec in, Vec out, Vec tmp, int r0, int n, double* partials);
   double reliable_update(int r);
+  std::vector<double> reliable_updates(int lo, int hi, const std::vector<char>& upd);
   void finish(int r, bool breakdown);
   void move_slot(int from, int to);
-  Vec at(Vec v, int r) const { return vec(static_cast<char*>(v.p) + (long)r * v.stride, v.stride); }
+  // slot r of a batch field; v must be the field's base (slot 0), never a slot view
+  Vec at(Vec v, int r) const;
   double norm2_slot(Vec v, int r);
   void clear_graphs();
 

@@ -124,6 +124,7 @@ struct HopArgs {
   int xcd_remap;                         // contiguous site ranges per XCD
   int sg;                                // sites per right-hand side group (set by the launcher)
   int ngroups;                           // blocks per site range (set by the launcher)
+  int gmajor;                            // 1: right-hand side group is the slowest block index
   int clov_shared;                       // every right-hand side uses clov[0]
   Walk walk;                             // site traversal order
 };

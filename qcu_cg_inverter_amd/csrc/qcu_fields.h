@@ -95,6 +95,14 @@ struct SpinorIO<F64> {
       v[c] = {q.x, q.y};
     }
   }
+  static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
+    const double2* b = static_cast<const double2*>(base) + i + (long)(6 * h) * vh;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double2 q = ld_nt(b + c * vh);
+      v[c] = {q.x, q.y};
+    }
+  }
   static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
     double2* b = static_cast<double2*>(base) + i;
 #pragma unroll
@@ -116,6 +124,15 @@ struct SpinorIO<F32> {
   }
   static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
     load_cached(base, vh, i, v);
+  }
+  static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
+    const float4* b = static_cast<const float4*>(base) + i + (long)(3 * h) * vh;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float4 q = b[k * vh];
+      v[2 * k] = {q.x, q.y};
+      v[2 * k + 1] = {q.z, q.w};
+    }
   }
   static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
     float4* b = static_cast<float4*>(base) + i;
@@ -162,6 +179,19 @@ struct SpinorIO<F16> {
   }
   static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
     load_cached(base, vh, i, v);
+  }
+  // components 0..5 are word 0 and the first half of word 1; 6..11 the rest
+  static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
+    const uint4* b = static_cast<const uint4*>(base) + i + (long)h * vh;
+    float nrm = norms(base, vh)[i];
+    float f[16];
+    unpack8(b[0], f);
+    unpack8(b[vh], f + 8);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const int o = h ? 2 * c + 4 : 2 * c;
+      v[c] = {f[o] * nrm, f[o + 1] * nrm};
+    }
   }
   static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
     float m = 0.f;
@@ -277,6 +307,16 @@ struct CloverIO<F64> {
       a[2 * k + 1] = q.y;
     }
   }
+  // chiral block b (36 reals) only
+  static __device__ __forceinline__ void load_block(const void* c, long vh, long i, int b, T m[36]) {
+    const double2* p = static_cast<const double2*>(c) + i + (long)(18 * b) * vh;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+      double2 q = ld_nt(p + k * vh);
+      m[2 * k] = q.x;
+      m[2 * k + 1] = q.y;
+    }
+  }
 };
 template <>
 struct CloverIO<F32> {
@@ -292,6 +332,17 @@ struct CloverIO<F32> {
       a[4 * k + 3] = q.w;
     }
   }
+  static __device__ __forceinline__ void load_block(const void* c, long vh, long i, int b, T m[36]) {
+    const float4* p = static_cast<const float4*>(c) + i + (long)(9 * b) * vh;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      float4 q = ld_nt(p + k * vh);
+      m[4 * k] = q.x;
+      m[4 * k + 1] = q.y;
+      m[4 * k + 2] = q.z;
+      m[4 * k + 3] = q.w;
+    }
+  }
 };
 template <>
 struct CloverIO<F16> {
@@ -301,32 +352,41 @@ struct CloverIO<F16> {
 #pragma unroll
     for (int k = 0; k < 9; ++k) unpack8(ld_nt(b + k * vh), a + 8 * k);
   }
+  // block 0 is halves 0..35 (words 0..4, first half of word 4), block 1 is 36..71
+  static __device__ __forceinline__ void load_block(const void* c, long vh, long i, int b, T m[36]) {
+    const uint4* p = static_cast<const uint4*>(c) + i + (long)(4 * b) * vh;
+    float f[40];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) unpack8(ld_nt(p + k * vh), f + 8 * k);
+#pragma unroll
+    for (int k = 0; k < 36; ++k) m[k] = b ? f[k + 4] : f[k];
+  }
 };
+
+// y = M x for one packed Hermitian 6x6 chiral block (6 real diagonals, then the 15
+// complex entries below the diagonal row by row)
+template <typename T>
+__device__ __forceinline__ void clover_mul_block(const T m[36], const cx<T> x[6], cx<T> y[6]) {
+#pragma unroll
+  for (int r = 0; r < 6; ++r) y[r] = m[r] * x[r];
+  int k = 6;
+#pragma unroll
+  for (int r = 1; r < 6; ++r) {
+#pragma unroll
+    for (int c = 0; c < r; ++c) {
+      cx<T> l = {m[k], m[k + 1]};
+      k += 2;
+      y[r] = y[r] + l * x[c];
+      y[c] = y[c] + conj(l) * x[r];
+    }
+  }
+}
 
 // out = blockdiag(M0, M1) v for one site, M packed as above
 template <typename T>
 __device__ __forceinline__ void clover_mul(const T a[72], const cx<T> v[12], cx<T> out[12]) {
 #pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const T* m = a + 36 * b;
-    const cx<T>* x = v + 6 * b;
-    cx<T> y[6];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) y[r] = m[r] * x[r];
-    int k = 6;
-#pragma unroll
-    for (int r = 1; r < 6; ++r) {
-#pragma unroll
-      for (int c = 0; c < r; ++c) {
-        cx<T> l = {m[k], m[k + 1]};
-        k += 2;
-        y[r] = y[r] + l * x[c];
-        y[c] = y[c] + conj(l) * x[r];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 6; ++r) out[6 * b + r] = y[r];
-  }
+  for (int b = 0; b < 2; ++b) clover_mul_block(a + 36 * b, v + 6 * b, out + 6 * b);
 }
 
 // XCD-aware block order: the hardware hands workgroup b to XCD (b mod 8); remap so

@@ -27,11 +27,14 @@ void load_tune_cache() {
   if (!path) return;
   FILE* f = fopen(path, "r");
   if (!f) return;
-  char key[128];
-  Tune t;
-  double ms;
-  while (fscanf(f, "%127s %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt, &ms) == 8)
-    ctx().tune[key] = t;
+  char line[256], key[128];
+  while (fgets(line, sizeof line, f)) {
+    Tune t;
+    double ms;
+    if (sscanf(line, "%127s %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt, &t.gm,
+               &ms) == 9)
+      ctx().tune[key] = t;
+  }
   fclose(f);
 }
 
@@ -40,7 +43,7 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, ms);
+  fprintf(f, "%s %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm, ms);
   fclose(f);
 }
 
@@ -56,14 +59,18 @@ static std::vector<Tune> candidates(int prec, int nrhs) {
           for (int rpt = 1; rpt <= 4; ++rpt) {
             if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
             if (prec == F64 && rpt > 1) continue;  // FP64 spills beyond one right-hand side per thread
-            Tune t;
-            t.block = block;
-            t.rb = rb;
-            t.rpt = rpt;
-            t.xcd = xcd;
-            t.tz = tl[0];
-            t.tt = tl[1];
-            out.push_back(t);
+            const bool groups = rb * rpt < nrhs;  // more than one block per site range
+            for (int gm = 0; gm <= (groups ? 1 : 0); ++gm) {
+              Tune t;
+              t.block = block;
+              t.rb = rb;
+              t.rpt = rpt;
+              t.xcd = xcd;
+              t.tz = tl[0];
+              t.tt = tl[1];
+              t.gm = gm;
+              out.push_back(t);
+            }
           }
         }
   return out;
@@ -103,6 +110,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   const int iters = vh * nrhs > (1 << 22) ? 10 : 30;
   for (const Tune& t : candidates(prec, nrhs)) {
     args.xcd_remap = t.xcd;
+    args.gmajor = t.gm;
     args.walk = make_walk(g.geo, t.tz, t.tt);
     if (t.tz && !args.walk.tile_z) continue;  // tile does not fit this lattice
     args.clov_shared = 1;
@@ -160,7 +168,39 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* s
       shape[3] = t.xcd;
       shape[4] = t.tz;
       shape[5] = t.tt;
+      shape[6] = t.gm;
     }
+    return QCU_SUCCESS;
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    return e.code;
+  }
+}
+
+extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, const int* shape) {
+  using namespace qcu;
+  try {
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    ensure_init();
+    QREQ(h && shape, "qcuSetHopShape: NULL argument");
+    QREQ(precision >= 0 && precision <= 2, "bad precision");
+    QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
+    QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
+    Tune t;
+    t.block = shape[0];
+    t.rb = shape[1];
+    t.rpt = shape[2];
+    t.xcd = shape[3];
+    t.tz = shape[4];
+    t.tt = shape[5];
+    t.gm = shape[6];
+    QREQ(t.block == 64 || t.block == 128 || t.block == 256, "block must be 64, 128 or 256");
+    QREQ(t.rb >= 1 && t.block % (64 * t.rb) == 0, "rb must divide block into 64-thread groups");
+    QREQ(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1), "rpt must be 1..4 (1 for FP64)");
+    QREQ((t.xcd == 0 || t.xcd == 1) && (t.gm == 0 || t.gm == 1), "xcd and group order are 0 or 1");
+    QREQ(t.tz >= 0 && t.tt >= 0 && (t.tz == 0) == (t.tt == 0), "tile extents must be both 0 or both positive");
+    GaugeField& g = *reinterpret_cast<GaugeField*>(h);
+    set_tuned_hop(precision, recon, g.geo.vh, nrhs, t);
     return QCU_SUCCESS;
   } catch (const Error& e) {
     set_last_error(e.code, e.what());
