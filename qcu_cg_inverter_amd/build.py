@@ -34,9 +34,12 @@ SOURCES = [
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# packed-FP32 instructions are disabled for all device code
+# packed-FP32 instructions are disabled for all device code.  No implicit FMA
+# contraction: where the kernels fuse, they say so (cx ops in qcu_fields.h), so every
+# instantiation of a kernel (right-hand sides per thread, epilogue, shape) rounds the
+# same way and batched results equal single ones bit for bit.
 CFLAGS = [
-    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,

@@ -209,10 +209,10 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
   auto it = ctx().tune.find(tune_key(prec, recon, vh, nrhs));
   if (it != ctx().tune.end()) return it->second;
   Tune t;
-  if (nrhs > 1) {  // wave groups of 64 sites; FP32/FP16 also carry several per thread
+  if (nrhs > 1) {  // wave groups of 64 sites sharing link loads, one right-hand side per thread
+    // (several per thread need more registers than the occupancy they cost is worth)
     t.rb = nrhs >= 4 ? 4 : nrhs;
-    t.rpt = prec == F64 ? 1 : (nrhs + t.rb - 1) / t.rb;
-    if (t.rpt > 4) t.rpt = 4;
+    t.rpt = 1;
     t.block = 64 * t.rb;
     t.tz = t.tt = 4;  // short neighbour reuse distances (ignored where tiles do not fit)
   }

@@ -11,6 +11,11 @@ struct cx {
   T re, im;
 };
 
+// Fused multiply-adds are written out (the library is built without implicit
+// contraction), so a kernel's rounding depends only on its source.
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
 template <typename T>
@@ -19,7 +24,7 @@ template <typename T>
 __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
 template <typename T>
 __device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return {fmx(a.re, b.re, -(a.im * b.im)), fmx(a.re, b.im, a.im * b.re)};
 }
 template <typename T>
 __device__ __forceinline__ cx<T> operator*(T s, cx<T> a) { return {s * a.re, s * a.im}; }
@@ -28,8 +33,21 @@ __device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
 // a * conj(b)
 template <typename T>
 __device__ __forceinline__ cx<T> mulc(cx<T> a, cx<T> b) {
-  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+  return {fmx(a.re, b.re, a.im * b.im), fmx(a.im, b.re, -(a.re * b.im))};
 }
+// c + a b
+template <typename T>
+__device__ __forceinline__ cx<T> cmad(cx<T> a, cx<T> b, cx<T> c) {
+  return {fmx(a.re, b.re, fmx(-a.im, b.im, c.re)), fmx(a.re, b.im, fmx(a.im, b.re, c.im))};
+}
+// c + conj(a) b
+template <typename T>
+__device__ __forceinline__ cx<T> cmadc(cx<T> a, cx<T> b, cx<T> c) {
+  return {fmx(a.re, b.re, fmx(a.im, b.im, c.re)), fmx(a.re, b.im, fmx(-a.im, b.re, c.im))};
+}
+// y + s a (real s)
+template <typename T>
+__device__ __forceinline__ cx<T> axpy(T s, cx<T> a, cx<T> y) { return {fmx(s, a.re, y.re), fmx(s, a.im, y.im)}; }
 // multiply by i^k (k = 0..3)
 template <typename T>
 __device__ __forceinline__ cx<T> iph(cx<T> a, int k) {
@@ -37,7 +55,7 @@ __device__ __forceinline__ cx<T> iph(cx<T> a, int k) {
   return k == 0 ? a : k == 1 ? cx<T>{-a.im, a.re} : k == 2 ? cx<T>{-a.re, -a.im} : cx<T>{a.im, -a.re};
 }
 template <typename T>
-__device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ T norm2(cx<T> a) { return fmx(a.re, a.re, a.im * a.im); }
 
 // streaming loads (nontemporal hint) of the HIP vector types through clang vectors
 typedef double v2d_t __attribute__((ext_vector_type(2)));
@@ -376,8 +394,8 @@ __device__ __forceinline__ void clover_mul_block(const T m[36], const cx<T> x[6]
     for (int c = 0; c < r; ++c) {
       cx<T> l = {m[k], m[k + 1]};
       k += 2;
-      y[r] = y[r] + l * x[c];
-      y[c] = y[c] + conj(l) * x[r];
+      y[r] = cmad(l, x[c], y[r]);
+      y[c] = cmadc(l, x[r], y[c]);
     }
   }
 }

@@ -56,9 +56,11 @@ static std::vector<Tune> candidates(int prec, int nrhs) {
       for (int block : blocks)
         for (int rb = 1; rb <= 4; rb *= 2) {
           if (block % (64 * rb) != 0 || rb > nrhs) continue;
-          for (int rpt = 1; rpt <= 4; ++rpt) {
+          for (int rpt = 1; rpt <= 2; ++rpt) {
             if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
-            if (prec == F64 && rpt > 1) continue;  // FP64 spills beyond one right-hand side per thread
+            // FP64 spills beyond one right-hand side per thread, and beyond two the FP32
+            // and FP16 kernels drop to one wave per SIMD
+            if (prec == F64 && rpt > 1) continue;
             const bool groups = rb * rpt < nrhs;  // more than one block per site range
             for (int gm = 0; gm <= (groups ? 1 : 0); ++gm) {
               Tune t;

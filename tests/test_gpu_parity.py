@@ -284,11 +284,15 @@ def test_tuned_shape_keeps_bits(q, prec):
         assert np.array_equal(a, b)
 
 
-def test_batch12_clover_mixed(q):
-    """12 right-hand sides in one batched FP32-inner clover solve (several per
-    thread): every one converges to 1e-10 and equals its own solve alone."""
+@pytest.mark.parametrize("shape", [None, (256, 4, 3, 1, 0, 0, 0), (128, 2, 2, 1, 0, 0, 1)])
+def test_batch12_clover_mixed(q, shape):
+    """12 right-hand sides in one batched FP32-inner clover solve (default launch
+    shape, or several right-hand sides per thread in either group order): every one
+    converges to 1e-10 and equals its own solve alone bit for bit."""
     L = (8, 8, 8, 8)
     g = q.Gauge.random(L, seed=33, eps=0.25)
+    if shape is not None:
+        q.set_hop_shape(g, 1, 12, 12, dict(zip(q.SHAPE_KEYS, shape)))
     op = q.Operator(0.122, 1.0, -1, 12)
     sp = q.default_solve_params(tol=1e-10, precision=1)
     bs = [orc.source_point(L, (0, 0, 0, 0), s, c) for s in range(4) for c in range(3)]
