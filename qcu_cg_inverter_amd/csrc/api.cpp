@@ -261,7 +261,7 @@ void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, 
   Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
   a.xcd_remap = t.xcd;
   a.gmajor = t.gm;
-  a.walk = make_walk(g.geo, t.tz, t.tt);
+  a.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }
 

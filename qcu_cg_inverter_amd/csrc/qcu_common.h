@@ -57,14 +57,16 @@ struct Geom {
 };
 
 // Order in which the hopping kernel walks the sites.  tile_z = 0: storage order.
-// Otherwise (z, t) is cut into tile_z x tile_t tiles walked one after another, and
-// inside a tile the xy-planes run with t fastest, so a site's t-neighbours are one
-// plane away and its z-neighbours tile_t planes away (instead of a whole t-slice):
-// the neighbour spinors of a batch of right-hand sides stay in the XCD's L2.  A wave
-// still covers 64 consecutive storage sites (the plane size is a multiple of 64).
+// Otherwise (y, z, t) is cut into tile_y x tile_z x tile_t tiles walked one after
+// another (y tiles fastest, then z, then t).  A tile is a stack of sub-planes (all x,
+// tile_y rows of y) run with t fastest, then z, so a site's t-neighbours are one
+// sub-plane away and its z-neighbours tile_t sub-planes away instead of a whole
+// t-slice: the neighbour spinors of a batch of right-hand sides stay in the XCD's L2.
+// A sub-plane is a run of consecutive storage sites whose length is a multiple of 64,
+// so a wave still covers one aligned 64-site storage chunk.
 struct Walk {
-  int tile_z, tile_t, ntz;
-  FastDiv dplane, dtt, dtz, dntz;
+  int tile_y, tile_z, tile_t, ny, ntz;
+  FastDiv dsp, dtt, dtz, dny, dntz;
 };
 
 inline Geom make_geom(const int L[4]) {
@@ -131,17 +133,21 @@ struct HopArgs {
 
 // walk with (tile_z, tile_t) tiles, or storage order when the tiles do not divide the
 // lattice or a plane is not a multiple of 64 sites
-inline Walk make_walk(const Geom& g, int tz, int tt) {
+inline Walk make_walk(const Geom& g, int ty, int tz, int tt) {
   Walk w;
-  w.tile_z = w.tile_t = w.ntz = 0;
-  long plane = (long)g.lxh * g.L[1];
-  if (tz <= 0 || tt <= 0 || plane % 64 != 0 || g.L[2] % tz != 0 || g.L[3] % tt != 0) return w;
+  w.tile_y = w.tile_z = w.tile_t = w.ny = w.ntz = 0;
+  if (ty <= 0) ty = g.L[1];
+  const long sub = (long)g.lxh * ty;
+  if (tz <= 0 || tt <= 0 || g.L[1] % ty != 0 || sub % 64 != 0 || g.L[2] % tz != 0 || g.L[3] % tt != 0) return w;
+  w.tile_y = ty;
   w.tile_z = tz;
   w.tile_t = tt;
+  w.ny = g.L[1] / ty;
   w.ntz = g.L[2] / tz;
-  w.dplane = FastDiv((uint32_t)plane);
+  w.dsp = FastDiv((uint32_t)sub);
   w.dtt = FastDiv((uint32_t)tt);
   w.dtz = FastDiv((uint32_t)tz);
+  w.dny = FastDiv((uint32_t)w.ny);
   w.dntz = FastDiv((uint32_t)w.ntz);
   return w;
 }

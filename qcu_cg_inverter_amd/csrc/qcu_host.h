@@ -65,8 +65,8 @@ struct GaugeField {
 
 // launch shape of the hopping kernel: `block` threads in `rb` wave groups that share
 // sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off, the
-// (z, t) tile of the site walk (0 = storage order), and whether the right-hand side
-// group is the slowest block index (gm) or the fastest
+// (y, z, t) tile of the site walk (tz = 0: storage order; ty = 0: whole xy-planes),
+// and whether the right-hand side group is the slowest block index (gm) or the fastest
 struct Tune {
   int block = 256;
   int rb = 1;
@@ -75,6 +75,7 @@ struct Tune {
   int tz = 0;
   int tt = 0;
   int gm = 0;
+  int ty = 0;
 };
 
 struct Context {

@@ -31,8 +31,8 @@ void load_tune_cache() {
   while (fgets(line, sizeof line, f)) {
     Tune t;
     double ms;
-    if (sscanf(line, "%127s %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt, &t.gm,
-               &ms) == 9)
+    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt, &t.gm,
+               &t.ty, &ms) == 10)
       ctx().tune[key] = t;
   }
   fclose(f);
@@ -43,24 +43,25 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm, ms);
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm, t.ty,
+          ms);
   fclose(f);
 }
 
 static std::vector<Tune> candidates(int prec, int nrhs) {
   std::vector<Tune> out;
   const int blocks[] = {64, 128, 256};
-  const int tiles[][2] = {{0, 0}, {4, 4}, {8, 8}};
+  // (ty, tz, tt) walk tiles; ty = 0 keeps whole xy-planes
+  const int tiles[][3] = {{0, 0, 0}, {0, 4, 4}, {0, 8, 8}, {4, 4, 4}, {8, 4, 4}, {16, 4, 4}, {4, 8, 8}, {8, 8, 8}, {4, 2, 4}};
   for (auto& tl : tiles)
     for (int xcd = 0; xcd <= 1; ++xcd)
       for (int block : blocks)
         for (int rb = 1; rb <= 4; rb *= 2) {
           if (block % (64 * rb) != 0 || rb > nrhs) continue;
-          for (int rpt = 1; rpt <= 2; ++rpt) {
+          for (int rpt = 1; rpt <= 1; ++rpt) {
+            // several right-hand sides per thread (qcuSetHopShape) spill in FP64 and
+            // drop FP32 / FP16 to one wave per SIMD, so the search keeps one
             if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
-            // FP64 spills beyond one right-hand side per thread, and beyond two the FP32
-            // and FP16 kernels drop to one wave per SIMD
-            if (prec == F64 && rpt > 1) continue;
             const bool groups = rb * rpt < nrhs;  // more than one block per site range
             for (int gm = 0; gm <= (groups ? 1 : 0); ++gm) {
               Tune t;
@@ -68,8 +69,9 @@ static std::vector<Tune> candidates(int prec, int nrhs) {
               t.rb = rb;
               t.rpt = rpt;
               t.xcd = xcd;
-              t.tz = tl[0];
-              t.tt = tl[1];
+              t.ty = tl[0];
+              t.tz = tl[1];
+              t.tt = tl[2];
               t.gm = gm;
               out.push_back(t);
             }
@@ -113,7 +115,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   for (const Tune& t : candidates(prec, nrhs)) {
     args.xcd_remap = t.xcd;
     args.gmajor = t.gm;
-    args.walk = make_walk(g.geo, t.tz, t.tt);
+    args.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
     if (t.tz && !args.walk.tile_z) continue;  // tile does not fit this lattice
     args.clov_shared = 1;
     for (int w = 0; w < 2; ++w) launch_hop(prec, recon, EPI_PLAIN, args, t.block, t.rb, t.rpt, stream());
@@ -171,6 +173,7 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* s
       shape[4] = t.tz;
       shape[5] = t.tt;
       shape[6] = t.gm;
+      shape[7] = t.ty;
     }
     return QCU_SUCCESS;
   } catch (const Error& e) {
@@ -196,11 +199,13 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     t.tz = shape[4];
     t.tt = shape[5];
     t.gm = shape[6];
+    t.ty = shape[7];
     QREQ(t.block == 64 || t.block == 128 || t.block == 256, "block must be 64, 128 or 256");
     QREQ(t.rb >= 1 && t.block % (64 * t.rb) == 0, "rb must divide block into 64-thread groups");
     QREQ(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1), "rpt must be 1..4 (1 for FP64)");
     QREQ((t.xcd == 0 || t.xcd == 1) && (t.gm == 0 || t.gm == 1), "xcd and group order are 0 or 1");
-    QREQ(t.tz >= 0 && t.tt >= 0 && (t.tz == 0) == (t.tt == 0), "tile extents must be both 0 or both positive");
+    QREQ(t.tz >= 0 && t.tt >= 0 && t.ty >= 0 && (t.tz == 0) == (t.tt == 0),
+         "tile extents must be non-negative, tz and tt both 0 or both positive");
     GaugeField& g = *reinterpret_cast<GaugeField*>(h);
     set_tuned_hop(precision, recon, g.geo.vh, nrhs, t);
     return QCU_SUCCESS;

@@ -311,12 +311,12 @@ def test_hop_shapes_keep_bits(q, prec):
     """Every launch shape of a 12-RHS hop -- block size, wave groups sharing sites,
     right-hand sides per thread, XCD order, walk tile, right-hand side groups as the
     fastest or the slowest block index -- gives the bits of single launches."""
-    L = (16, 8, 8, 16)  # 64-site planes: the 4x4 and 8x8 walk tiles apply
+    L = (16, 16, 8, 16)  # 128-site planes: whole-plane and 8-row (y, z, t) walk tiles apply
     g = q.Gauge.random(L, seed=34, eps=0.3)
     vs = [orc.source_gauss(L, 80 + r)[1] for r in range(12)]
     singles = [q.hop(g, v, 0, dagger=0, t_boundary=-1, recon=12, precision=prec) for v in vs]
-    shapes = [(256, 4, 1, 1, 4, 4, 1), (128, 2, 3, 1, 4, 4, 1), (64, 1, 2, 0, 0, 0, 1),
-              (256, 2, 1, 1, 8, 8, 0), (128, 1, 4, 0, 4, 4, 0)]
+    shapes = [(256, 4, 1, 1, 4, 4, 1, 0), (128, 2, 3, 1, 4, 4, 1, 8), (64, 1, 2, 0, 0, 0, 1, 0),
+              (256, 2, 1, 1, 8, 8, 0, 8), (128, 1, 4, 0, 4, 4, 0, 0), (256, 4, 1, 1, 2, 4, 0, 8)]
     for sh in shapes:
         shape = dict(zip(q.SHAPE_KEYS, sh))
         if prec == 0:
