@@ -350,6 +350,8 @@ def main():
                     help="requests of the stream re-run as plain FP64 one-at-a-time solves")
     ap.add_argument("--kernels-12rhs", action="store_true",
                     help="profiler driver: copy kernel, then the 32^3x64 FP32 hop with 1 and 12 right-hand sides")
+    ap.add_argument("--section", choices=["config1", "config3", "config4", "serving", "dslash_12rhs"],
+                    help="profiler driver: run one section only and print its JSON")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiler driver: 1 GiB copy kernel x5 then the 16^3x32 FP64 hop x steps, nothing else")
     args = ap.parse_args()
@@ -368,6 +370,12 @@ def main():
         gk = q.Gauge.random(L_MAIN, seed=1, eps=0.2)
         ms = q.bench_hop(gk, precision=0, recon=18, nrhs=1, iters=args.steps)
         log(f"kernels-only: hop f64 r18 16^3x32 {ms*1e3:.2f} us/launch over {args.steps}")
+        return 0
+    if args.section:
+        fn = {"config1": config1_gpu, "config3": config3_section, "config4": config4_section,
+              "serving": lambda q_: serving_section(q_, args.serving_requests, args.serving_baseline),
+              "dslash_12rhs": multi_rhs_section}[args.section]
+        print(json.dumps({args.section: fn(q)}))
         return 0
     if args.kernels_12rhs:
         # profiler driver: 1 GiB copy x5, then the 32^3x64 FP32 recon-12 hop with 1
