@@ -395,8 +395,8 @@ def main():
             key = f"hop:1:12:{q.half_volume(gk.L)}:12"
             with open(cache) as f:
                 rows = [ln.split() for ln in f if ln.startswith(key + " ")]
-            sh = dict(zip(q.SHAPE_KEYS, map(int, rows[-1][1:9])))
-            if sh["rb"] * sh["rpt"] < 12:
+            sh = dict(zip(q.SHAPE_KEYS, map(int, rows[-1][1:10])))
+            if sh["rb"] * sh["rpw"] * sh["rpt"] < 12:
                 sh["group_major"] ^= 1
                 q.set_hop_shape(gk, 1, 12, 12, sh)
                 ms = q.bench_hop(gk, precision=1, recon=12, nrhs=12, iters=args.steps)

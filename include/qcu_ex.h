@@ -188,14 +188,15 @@ int qcuServerDestroy(qcuServer s);
 /* ---- tuning ------------------------------------------------------------------ */
 /* Times every launch shape of the hopping kernel for (precision, recon, nrhs) on this
  * gauge's volume and keeps the fastest for later launches.  shape (may be NULL)
- * receives 8 ints: threads per block, wave groups per block sharing sites, right-hand
+ * receives 9 ints: threads per block, wave groups per block sharing sites, right-hand
  * sides per thread, XCD-contiguous block order (0/1), the z and t extents of the
  * site-walk tile (0 = storage order), right-hand side groups as the slowest block
- * index (1) or the fastest (0), and the y extent of the walk tile (0 = all of y).
+ * index (1) or the fastest (0), the y extent of the walk tile (0 = all of y), and
+ * right-hand sides per wave (1, 2 or 4; a wave then covers 64, 32 or 16 sites).
  * A tile that does not divide the lattice falls back to storage order.  QCU_TUNE=1 makes solves tune on first use;
  * QCU_TUNE_CACHE=<file> persists results across processes. */
 int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* shape, double* ms);
-/* Sets the launch shape (the 8 ints above) used for (precision, recon, nrhs) on this
+/* Sets the launch shape (the 9 ints above) used for (precision, recon, nrhs) on this
  * gauge's volume, e.g. one tuned earlier.  Every valid shape gives the same bits. */
 int qcuSetHopShape(qcuGauge g, int precision, int recon, int nrhs, const int* shape);
 

@@ -209,9 +209,12 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
   auto it = ctx().tune.find(tune_key(prec, recon, vh, nrhs));
   if (it != ctx().tune.end()) return it->second;
   Tune t;
-  if (nrhs > 1) {  // wave groups of 64 sites sharing link loads, one right-hand side per thread
-    // (several per thread need more registers than the occupancy they cost is worth)
-    t.rb = nrhs >= 4 ? 4 : nrhs;
+  if (nrhs > 1) {  // waves of 16 sites x 4 right-hand sides (or 32 x 2) sharing link loads,
+    // one right-hand side per thread (several per thread need more registers than the
+    // occupancy they cost is worth), up to 4 such waves per block on the same sites
+    t.rpw = nrhs >= 4 ? 4 : 2;
+    t.rb = (nrhs + t.rpw - 1) / t.rpw;
+    if (t.rb > 4) t.rb = 4;
     t.rpt = 1;
     t.block = 64 * t.rb;
     t.tz = t.tt = 4;  // short neighbour reuse distances (ignored where tiles do not fit)
@@ -261,6 +264,7 @@ void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, 
   Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
   a.xcd_remap = t.xcd;
   a.gmajor = t.gm;
+  a.rpw = t.rpw;
   a.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }

@@ -2,8 +2,8 @@
 //
 // The reference does each BLAS-1 op as its own kernel plus a host round trip per
 // reduction (src/qcu_complex_computation.cu:10-134, src/cg_inverter.cu:181-256).
-// Here reductions are per-64-site wave partials (fixed butterfly order) summed by a
-// one-block kernel in a fixed order, so every result is bitwise reproducible and
+// Here reductions are per-16-site partials (fixed butterfly order within 16 lanes)
+// summed by a one-block kernel in a fixed order, so every result is bitwise reproducible and
 // independent of how right-hand sides are batched, and all scalars stay on the
 // device: a CG iteration needs no host synchronisation.
 #include "qcu_fields.h"
@@ -13,16 +13,17 @@ namespace qcu {
 
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-__device__ __forceinline__ double wave_sum64(double v) {
+// the same 16-lane butterfly as the hopping kernel's partials
+__device__ __forceinline__ double group16_sum(double v) {
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 16);
   return v;
 }
 
-// write the wave's partial sum (lane 0) at slot site/64
+// write each 16-lane group's partial sum (its first lane) at slot site/16
 __device__ __forceinline__ void put_partial(double* part, long vh, int r, long i, double v) {
-  v = wave_sum64(v);
-  if ((threadIdx.x & 63) == 0 && i < vh) part[(long)r * n_partials(vh) + i / 64] = v;
+  v = group16_sum(v);
+  if ((threadIdx.x & 15) == 0 && i < vh) part[(long)r * n_partials(vh) + i / 16] = v;
 }
 
 template <int P>

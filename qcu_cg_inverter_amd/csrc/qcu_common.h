@@ -127,6 +127,7 @@ struct HopArgs {
   int sg;                                // sites per right-hand side group (set by the launcher)
   int ngroups;                           // blocks per site range (set by the launcher)
   int gmajor;                            // 1: right-hand side group is the slowest block index
+  int rpw;                               // right-hand sides per wave (1, 2 or 4)
   int clov_shared;                       // every right-hand side uses clov[0]
   Walk walk;                             // site traversal order
 };

@@ -66,7 +66,8 @@ struct GaugeField {
 // launch shape of the hopping kernel: `block` threads in `rb` wave groups that share
 // sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off, the
 // (y, z, t) tile of the site walk (tz = 0: storage order; ty = 0: whole xy-planes),
-// and whether the right-hand side group is the slowest block index (gm) or the fastest
+// whether the right-hand side group is the slowest block index (gm) or the fastest,
+// and `rpw` right-hand sides per wave (lanes of a wave sharing sites)
 struct Tune {
   int block = 256;
   int rb = 1;
@@ -76,6 +77,7 @@ struct Tune {
   int tt = 0;
   int gm = 0;
   int ty = 0;
+  int rpw = 1;
 };
 
 struct Context {

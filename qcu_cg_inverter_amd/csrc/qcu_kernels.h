@@ -13,12 +13,12 @@ struct Vec {
 };
 inline Vec vec(void* p, long stride = 0) { return Vec{p, stride}; }
 
-// number of 64-site reduction partials of one half lattice
-__host__ __device__ inline long n_partials(long vh) { return (vh + 63) / 64; }
+// number of 16-site reduction partials of one half lattice
+__host__ __device__ inline long n_partials(long vh) { return (vh + 15) / 16; }
 
 // ------------------------------------------------------------------ hop -----
-// block threads = sg * rb; rb groups of sg sites share links; rpt right-hand sides
-// per thread (1..4)
+// block threads = sg * rb; the rb groups share sites (and link loads); a.rpw
+// right-hand sides per wave (1, 2 or 4), rpt per thread (1..4)
 void launch_hop(int prec, int recon, int epi, const HopArgs& a, int block, int rb, int rpt, hipStream_t st);
 
 // --------------------------------------------------------------- layout -----

@@ -31,8 +31,8 @@ void load_tune_cache() {
   while (fgets(line, sizeof line, f)) {
     Tune t;
     double ms;
-    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt, &t.gm,
-               &t.ty, &ms) == 10)
+    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt,
+               &t.gm, &t.ty, &t.rpw, &ms) == 11)
       ctx().tune[key] = t;
   }
   fclose(f);
@@ -43,31 +43,35 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm, t.ty,
-          ms);
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm,
+          t.ty, t.rpw, ms);
   fclose(f);
 }
 
+// Several right-hand sides per thread (qcuSetHopShape) spill in FP64 and drop FP32 /
+// FP16 to one wave per SIMD, so the search keeps one per thread and varies how many
+// share a wave (rpw) and a block (rb).
 static std::vector<Tune> candidates(int prec, int nrhs) {
+  (void)prec;
   std::vector<Tune> out;
-  const int blocks[] = {64, 128, 256};
+  const int blocks[] = {64, 128, 192, 256};
   // (ty, tz, tt) walk tiles; ty = 0 keeps whole xy-planes
-  const int tiles[][3] = {{0, 0, 0}, {0, 4, 4}, {0, 8, 8}, {4, 4, 4}, {8, 4, 4}, {16, 4, 4}, {4, 8, 8}, {8, 8, 8}, {4, 2, 4}};
+  const int tiles[][3] = {{0, 0, 0}, {0, 4, 4}, {0, 8, 8}, {8, 4, 4}, {4, 2, 4}};
   for (auto& tl : tiles)
     for (int xcd = 0; xcd <= 1; ++xcd)
       for (int block : blocks)
-        for (int rb = 1; rb <= 4; rb *= 2) {
-          if (block % (64 * rb) != 0 || rb > nrhs) continue;
-          for (int rpt = 1; rpt <= 1; ++rpt) {
-            // several right-hand sides per thread (qcuSetHopShape) spill in FP64 and
-            // drop FP32 / FP16 to one wave per SIMD, so the search keeps one
-            if (rb * rpt > nrhs + rb - 1) continue;  // every wave of a full block has work
-            const bool groups = rb * rpt < nrhs;  // more than one block per site range
+        for (int rb = 1; rb <= 4; ++rb) {
+          if (block % (64 * rb) != 0) continue;
+          for (int rpw = 1; rpw <= 4; rpw *= 2) {
+            const int slots = rb * rpw;  // right-hand sides one block serves
+            if (rpw > nrhs || slots > nrhs + rpw - 1) continue;  // every wave of a full block has work
+            const bool groups = slots < nrhs;  // more than one block per site range
             for (int gm = 0; gm <= (groups ? 1 : 0); ++gm) {
               Tune t;
               t.block = block;
               t.rb = rb;
-              t.rpt = rpt;
+              t.rpt = 1;
+              t.rpw = rpw;
               t.xcd = xcd;
               t.ty = tl[0];
               t.tz = tl[1];
@@ -115,6 +119,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   for (const Tune& t : candidates(prec, nrhs)) {
     args.xcd_remap = t.xcd;
     args.gmajor = t.gm;
+    args.rpw = t.rpw;
     args.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
     if (t.tz && !args.walk.tile_z) continue;  // tile does not fit this lattice
     args.clov_shared = 1;
@@ -174,6 +179,7 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* s
       shape[5] = t.tt;
       shape[6] = t.gm;
       shape[7] = t.ty;
+      shape[8] = t.rpw;
     }
     return QCU_SUCCESS;
   } catch (const Error& e) {
@@ -200,7 +206,9 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     t.tt = shape[5];
     t.gm = shape[6];
     t.ty = shape[7];
-    QREQ(t.block == 64 || t.block == 128 || t.block == 256, "block must be 64, 128 or 256");
+    t.rpw = shape[8];
+    QREQ(t.block == 64 || t.block == 128 || t.block == 192 || t.block == 256, "block must be 64, 128, 192 or 256");
+    QREQ(t.rpw == 1 || t.rpw == 2 || t.rpw == 4, "right-hand sides per wave must be 1, 2 or 4");
     QREQ(t.rb >= 1 && t.block % (64 * t.rb) == 0, "rb must divide block into 64-thread groups");
     QREQ(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1), "rpt must be 1..4 (1 for FP64)");
     QREQ((t.xcd == 0 || t.xcd == 1) && (t.gm == 0 || t.gm == 1), "xcd and group order are 0 or 1");
