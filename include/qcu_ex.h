@@ -187,7 +187,9 @@ int qcuServerDestroy(qcuServer s);
 
 /* ---- tuning ------------------------------------------------------------------ */
 /* Times every launch shape of the hopping kernel for (precision, recon, nrhs) on this
- * gauge's volume and keeps the fastest for later launches.  shape (may be NULL)
+ * gauge's volume and keeps the fastest for later launches; the eight fastest are
+ * re-timed with each epilogue that reads x or a clover field, and solves use the
+ * winner per epilogue.  shape (may be NULL; the plain epilogue's winner)
  * receives 9 ints: threads per block, wave groups per block sharing sites, right-hand
  * sides per thread, XCD-contiguous block order (0/1), the z and t extents of the
  * site-walk tile (0 = storage order), right-hand side groups as the slowest block
@@ -196,7 +198,7 @@ int qcuServerDestroy(qcuServer s);
  * A tile that does not divide the lattice falls back to storage order.  QCU_TUNE=1 makes solves tune on first use;
  * QCU_TUNE_CACHE=<file> persists results across processes. */
 int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* shape, double* ms);
-/* Sets the launch shape (the 9 ints above) used for (precision, recon, nrhs) on this
+/* Sets the launch shape (the 9 ints above, for every epilogue) used for (precision, recon, nrhs) on this
  * gauge's volume, e.g. one tuned earlier.  Every valid shape gives the same bits. */
 int qcuSetHopShape(qcuGauge g, int precision, int recon, int nrhs, const int* shape);
 

@@ -200,13 +200,19 @@ std::unique_ptr<GaugeField> make_gauge_random(const int L[4], uint64_t seed, dou
 }
 
 // ------------------------------------------------------------- tuning -----
-static std::string tune_key(int prec, int recon, long vh, int nrhs) {
+// "hop:prec:recon:vh:nrhs" for the plain epilogue, with ":e<epi>" for the others
+std::string tune_key(int prec, int recon, long vh, int nrhs, int epi) {
   char b[96];
-  snprintf(b, sizeof b, "hop:%d:%d:%ld:%d", prec, recon, vh, nrhs);
+  if (epi == EPI_PLAIN) snprintf(b, sizeof b, "hop:%d:%d:%ld:%d", prec, recon, vh, nrhs);
+  else snprintf(b, sizeof b, "hop:%d:%d:%ld:%d:e%d", prec, recon, vh, nrhs, epi);
   return b;
 }
-Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
-  auto it = ctx().tune.find(tune_key(prec, recon, vh, nrhs));
+// the shape tuned for this epilogue, else the one tuned for the plain epilogue, else
+// the default
+Tune tuned_hop(int prec, int recon, long vh, int nrhs, int epi) {
+  auto it = ctx().tune.find(tune_key(prec, recon, vh, nrhs, epi));
+  if (it != ctx().tune.end()) return it->second;
+  it = ctx().tune.find(tune_key(prec, recon, vh, nrhs, EPI_PLAIN));
   if (it != ctx().tune.end()) return it->second;
   Tune t;
   if (nrhs > 1) {  // waves of 16 sites x 4 right-hand sides (or 32 x 2) sharing link loads,
@@ -222,7 +228,8 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs) {
   return t;
 }
 void set_tuned_hop(int prec, int recon, long vh, int nrhs, const Tune& t) {
-  ctx().tune[tune_key(prec, recon, vh, nrhs)] = t;
+  for (int e = EPI_PLAIN; e <= EPI_CLOV_XPAY; ++e) ctx().tune.erase(tune_key(prec, recon, vh, nrhs, e));
+  ctx().tune[tune_key(prec, recon, vh, nrhs, EPI_PLAIN)] = t;
 }
 
 void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
@@ -261,7 +268,7 @@ void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, 
   a.dagger = dagger;
   a.tbc = tbc;
   a.nrhs = nrhs;
-  Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs);
+  Tune t = tuned_hop(prec, recon, g.geo.vh, nrhs, epi);
   a.xcd_remap = t.xcd;
   a.gmajor = t.gm;
   a.rpw = t.rpw;

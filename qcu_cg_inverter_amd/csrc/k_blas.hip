@@ -54,23 +54,31 @@ __global__ void norm2_kernel(Vec v, long vh, double* part) {
   put_partial(part, vh, r, i, s);
 }
 
-// one block of 256 threads per right-hand side; strided sequential sums then a
-// fixed LDS tree
+// one block of kSumThreads threads per right-hand side: every thread adds its
+// strided partials in eight independent chains (so eight loads are in flight), the
+// chains are combined in a fixed order, then a fixed LDS tree; the order depends on
+// nw only, so the sum of a right-hand side never depends on the batch
+constexpr int kSumThreads = 1024;
 __device__ __forceinline__ double block_sum_partials(const double* part, long nw) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (long w = threadIdx.x; w < nw; w += 256) s += part[w];
-  red[threadIdx.x] = s;
+  __shared__ double red[kSumThreads];
+  double c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  long w = threadIdx.x;
+  for (; w + 7 * kSumThreads < nw; w += 8 * kSumThreads) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] += part[w + k * kSumThreads];
+  }
+  for (; w < nw; w += kSumThreads) c[0] += part[w];
+  red[threadIdx.x] = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
   __syncthreads();
 #pragma unroll
-  for (int off = 128; off >= 1; off >>= 1) {
+  for (int off = kSumThreads / 2; off >= 1; off >>= 1) {
     if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
   return red[0];
 }
 
-__global__ void sum_partials_kernel(const double* part, long nw, double* out) {
+__global__ void __launch_bounds__(kSumThreads) sum_partials_kernel(const double* part, long nw, double* out) {
   double s = block_sum_partials(part + (long)blockIdx.x * nw, nw);
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
@@ -105,7 +113,7 @@ __global__ void clover_apply_kernel(const void* clov, Vec in, Vec out, long vh) 
 }
 
 // ---------------------------------------------------------------- CGLS ------
-__global__ void cg_beta_kernel(const double* part, long nw, CgScal* sc) {
+__global__ void __launch_bounds__(kSumThreads) cg_beta_kernel(const double* part, long nw, CgScal* sc) {
   CgScal& s = sc[blockIdx.x];
   if (!s.active) return;
   double g = block_sum_partials(part + (long)blockIdx.x * nw, nw);
@@ -120,7 +128,7 @@ __global__ void cg_beta_kernel(const double* part, long nw, CgScal* sc) {
   }
 }
 
-__global__ void cg_alpha_kernel(const double* part, long nw, CgScal* sc) {
+__global__ void __launch_bounds__(kSumThreads) cg_alpha_kernel(const double* part, long nw, CgScal* sc) {
   CgScal& s = sc[blockIdx.x];
   if (!s.active) return;
   double tt = block_sum_partials(part + (long)blockIdx.x * nw, nw);
@@ -137,7 +145,7 @@ __global__ void cg_alpha_kernel(const double* part, long nw, CgScal* sc) {
   }
 }
 
-__global__ void cg_ss_kernel(const double* part, long nw, CgScal* sc) {
+__global__ void __launch_bounds__(kSumThreads) cg_ss_kernel(const double* part, long nw, CgScal* sc) {
   CgScal& s = sc[blockIdx.x];
   if (!s.active) return;
   double ss = block_sum_partials(part + (long)blockIdx.x * nw, nw);
@@ -244,7 +252,7 @@ void launch_norm2(int prec, Vec v, long vh, int nrhs, double* part, hipStream_t 
   QCU_PSW(prec, norm2_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, v, vh, part);
 }
 void launch_sum_partials(const double* part, long nw, int nrhs, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, out);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(nrhs), dim3(kSumThreads), 0, st, part, nw, out);
 }
 void launch_axpby(int prec, double a, Vec x, double b, Vec y, long vh, int nrhs, hipStream_t st) {
   QCU_PSW(prec, axpby_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, a, x, b, y, vh);
@@ -253,13 +261,13 @@ void launch_clover_apply(int prec, const void* clov, Vec in, Vec out, long vh, i
   QCU_PSW(prec, clover_apply_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, clov, in, out, vh);
 }
 void launch_cg_beta(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
-  hipLaunchKernelGGL(cg_beta_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, sc);
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(nrhs), dim3(kSumThreads), 0, st, part, nw, sc);
 }
 void launch_cg_alpha(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
-  hipLaunchKernelGGL(cg_alpha_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, sc);
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(nrhs), dim3(kSumThreads), 0, st, part, nw, sc);
 }
 void launch_cg_ss(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
-  hipLaunchKernelGGL(cg_ss_kernel, dim3(nrhs), dim3(256), 0, st, part, nw, sc);
+  hipLaunchKernelGGL(cg_ss_kernel, dim3(nrhs), dim3(kSumThreads), 0, st, part, nw, sc);
 }
 void launch_cg_xpay(int prec, Vec r, Vec p, const CgScal* sc, long vh, int nrhs, hipStream_t st) {
   QCU_PSW(prec, cg_xpay_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, r, p, sc, vh);

@@ -104,7 +104,8 @@ void validate_lattice(const int L[4]);
 std::unique_ptr<GaugeField> make_gauge_from_aos(const void* aos_device, const int L[4]);
 std::unique_ptr<GaugeField> make_gauge_random(const int L[4], uint64_t seed, double eps);
 
-Tune tuned_hop(int prec, int recon, long vh, int nrhs);
+Tune tuned_hop(int prec, int recon, long vh, int nrhs, int epi = EPI_PLAIN);
+std::string tune_key(int prec, int recon, long vh, int nrhs, int epi);
 void set_tuned_hop(int prec, int recon, long vh, int nrhs, const Tune& t);
 // autotuner (tune.cpp)
 void load_tune_cache();

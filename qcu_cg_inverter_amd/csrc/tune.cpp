@@ -2,9 +2,12 @@
 //
 // For one (precision, link storage, half volume, right-hand sides) it times every
 // candidate shape -- block size, wave groups per block sharing sites, right-hand
-// sides per thread, XCD-contiguous block order -- on scratch fields with hipEvents
-// and keeps the fastest.  Results live in the context's table and, when
+// sides per wave, group order, XCD-contiguous block order, walk tile -- on scratch
+// fields with hipEvents and the plain epilogue, keeps the fastest, and re-times the
+// eight fastest with each epilogue that reads x or a clover field (those add loads
+// that can favour another shape).  Results live in the context's table and, when
 // QCU_TUNE_CACHE names a file, are appended there and reloaded by later processes.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -84,11 +87,6 @@ static std::vector<Tune> candidates(int prec, int nrhs) {
   return out;
 }
 
-static std::string key_of(int prec, int recon, long vh, int nrhs) {
-  char b[96];
-  snprintf(b, sizeof b, "hop:%d:%d:%ld:%d", prec, recon, vh, nrhs);
-  return b;
-}
 
 Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms) {
   load_tune_cache();
@@ -99,52 +97,78 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   DevBuf* b = ctx().buf("tune_b", fb * nrhs);
   launch_source_gauss(src->p, vh * 12 * nrhs, 777, stream());
   launch_spinor_from_aos(prec, src->p, vh * 192, vec(a->p, (long)fb), vh, nrhs, stream());
+  // a clover-sized scratch field for the clover epilogues (its values do not matter)
+  DevBuf* cl = ctx().buf("tune_clov", clover_bytes(prec, vh) / 2);
+  QHIP(hipMemsetAsync(cl->p, 0, clover_bytes(prec, vh) / 2, stream()));
   const void* gauge = g.get(prec, recon);
   HopArgs args;
   memset(&args, 0, sizeof args);
   args.in = a->p;
   args.out = b->p;
-  args.in_stride = args.out_stride = (long)fb;
+  args.x = a->p;
+  args.in_stride = args.out_stride = args.x_stride = (long)fb;
   args.gauge = gauge;
-  for (int r = 0; r < nrhs; ++r) args.coef[r] = 1.0;
+  for (int r = 0; r < nrhs; ++r) {
+    args.coef[r] = 1.0;
+    args.clov[r] = cl->p;
+  }
+  args.clov_shared = 1;
   args.geo = g.geo;
   args.tbc = -1.0;
   args.nrhs = nrhs;
   hipEvent_t e0, e1;
   QHIP(hipEventCreate(&e0));
   QHIP(hipEventCreate(&e1));
-  Tune best;
-  double bms = 1e30;
   const int iters = vh * nrhs > (1 << 22) ? 10 : 30;
-  for (const Tune& t : candidates(prec, nrhs)) {
+  auto time_shape = [&](const Tune& t, int epi) -> double {
     args.xcd_remap = t.xcd;
     args.gmajor = t.gm;
     args.rpw = t.rpw;
     args.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
-    if (t.tz && !args.walk.tile_z) continue;  // tile does not fit this lattice
-    args.clov_shared = 1;
-    for (int w = 0; w < 2; ++w) launch_hop(prec, recon, EPI_PLAIN, args, t.block, t.rb, t.rpt, stream());
+    for (int w = 0; w < 2; ++w) launch_hop(prec, recon, epi, args, t.block, t.rb, t.rpt, stream());
     QHIP(hipEventRecord(e0, stream()));
     for (int i = 0; i < iters; ++i) {
       args.parity = i & 1;
-      launch_hop(prec, recon, EPI_PLAIN, args, t.block, t.rb, t.rpt, stream());
+      launch_hop(prec, recon, epi, args, t.block, t.rb, t.rpt, stream());
     }
     QHIP(hipEventRecord(e1, stream()));
     QHIP(hipEventSynchronize(e1));
     QHIP(hipGetLastError());
     float ms = 0.f;
     QHIP(hipEventElapsedTime(&ms, e0, e1));
-    double per = (double)ms / iters;
-    if (per < bms) {
-      bms = per;
-      best = t;
+    return (double)ms / iters;
+  };
+  // every candidate with the plain epilogue ...
+  std::vector<std::pair<double, Tune>> timed;
+  for (const Tune& t : candidates(prec, nrhs)) {
+    if (t.tz && !make_walk(g.geo, t.ty, t.tz, t.tt).tile_z) continue;  // tile does not fit this lattice
+    timed.push_back({time_shape(t, EPI_PLAIN), t});
+  }
+  std::stable_sort(timed.begin(), timed.end(),
+                   [](const std::pair<double, Tune>& x, const std::pair<double, Tune>& y) { return x.first < y.first; });
+  const Tune best = timed.front().second;
+  const double bms = timed.front().first;
+  std::string key = tune_key(prec, recon, vh, nrhs, EPI_PLAIN);
+  ctx().tune[key] = best;
+  append_cache(key, best, bms);
+  // ... then the fastest few again with each epilogue that reads x or the clover
+  const size_t top = timed.size() < 8 ? timed.size() : 8;
+  for (int epi = EPI_XPAY; epi <= EPI_CLOV_XPAY; ++epi) {
+    Tune be = best;
+    double bt = 1e30;
+    for (size_t k = 0; k < top; ++k) {
+      double ms = time_shape(timed[k].second, epi);
+      if (ms < bt) {
+        bt = ms;
+        be = timed[k].second;
+      }
     }
+    std::string ke = tune_key(prec, recon, vh, nrhs, epi);
+    ctx().tune[ke] = be;
+    append_cache(ke, be, bt);
   }
   QHIP(hipEventDestroy(e0));
   QHIP(hipEventDestroy(e1));
-  std::string key = key_of(prec, recon, vh, nrhs);
-  ctx().tune[key] = best;
-  append_cache(key, best, bms);
   if (best_ms) *best_ms = bms;
   return best;
 }
@@ -154,7 +178,7 @@ void maybe_autotune(GaugeField& g, int prec, int recon, int nrhs) {
   load_tune_cache();
   const char* e = getenv("QCU_TUNE");
   if (!(e && strcmp(e, "0") != 0)) return;
-  if (ctx().tune.count(key_of(prec, recon, g.geo.vh, nrhs))) return;
+  if (ctx().tune.count(tune_key(prec, recon, g.geo.vh, nrhs, EPI_PLAIN))) return;
   autotune_hop(g, prec, recon, nrhs, nullptr);
 }
 
