@@ -95,15 +95,15 @@ def config3_section(q):
     return out
 
 
-def serving_section(q, n_requests=240, n_baseline=12, seed=2024):
+def serving_section(q, n_requests=240, seed=2024):
     """Poisson stream of solve requests over 8 resident gauges (2 per volume in
     {8^4, 16^3x32, 24^3x48, 32^3x64}); each request draws kappa ~ U[0.11, 0.124],
     Wilson or clover (c_sw = 1.0), 1-12 Gaussian right-hand sides and tol in
     {1e-8, 1e-10}.  Capacity is measured with a closed burst, arrivals then run at
     0.8x capacity through the continuous-batching server (FP32 inner, FP64 reliable
-    updates).  The baseline is the plain FP64 solve, one right-hand side at a time,
-    timed on a sample of the same requests and replayed through a FIFO queue with the
-    same arrival rate."""
+    updates).  The baseline is the plain FP64 solve, one right-hand side at a time:
+    one solve per request class is timed and the whole stream is replayed through a
+    FIFO queue with the same arrival times."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -164,34 +164,38 @@ def serving_section(q, n_requests=240, n_baseline=12, seed=2024):
            "all_converged": all(s.converged for s in sts),
            "max_true_residual_over_tol": max(s.true_residual / r["tol"] for s, r in zip(sts, reqs)),
            "peak_device_bytes": usage["peak_bytes"]}
-    # baseline: plain FP64, one right-hand side at a time, on a sample of the stream
-    sample = reqs[:n_baseline]
-    service = []
+    # baseline: plain FP64, one right-hand side at a time.  One solve is timed per
+    # class (gauge, Wilson or clover, tol) at kappa = 0.117, then the whole stream is
+    # replayed through a FIFO queue with the same arrival times, each request taking
+    # nrhs times its class's solve time
     sp = q.default_solve_params(tol=1e-10, precision=0, graph=0)
-    for r in sample:
-        L = vols[r["g"] // 2]
-        op = q.Operator(r["kappa"], r["csw"], -1, 12)
-        sp.tol = r["tol"]
+    cls_time = {}
+    for gi in range(8):
+        L = vols[gi // 2]
         x = q.DeviceBuffer(2 * q.half_volume(L) * 192)
-        tsum = 0.0
-        for k in range(r["nrhs"]):
-            b = q.source_gaussian_device(L, r["seed"] + k)
-            t1 = time.perf_counter()
-            q.solve_batch_device(gauges[r["g"]], op, [x], [b], sp)
-            tsum += time.perf_counter() - t1
-            b.free()
-        service.append(tsum)
+        b = q.source_gaussian_device(L, 4242 + gi)
+        for csw in (0.0, 1.0):
+            for tol in (1e-8, 1e-10):
+                sp.tol = tol
+                op = q.Operator(0.117, csw, -1, 12)
+                q.solve_batch_device(gauges[gi], op, [x], [b], sp)  # clover and buffers built
+                t1 = time.perf_counter()
+                q.solve_batch_device(gauges[gi], op, [x], [b], sp)
+                cls_time[(gi, csw, tol)] = time.perf_counter() - t1
+        b.free()
+        x.free()
+    service = [r["nrhs"] * cls_time[(r["g"], r["csw"], r["tol"])] for r in reqs]
     busy_until, blat = 0.0, []
-    for ta, s in zip(arrivals[:n_baseline], service):
+    for ta, s in zip(arrivals, service):
         start = max(ta, busy_until)
         busy_until = start + s
         blat.append(busy_until - ta)
     blat = np.array(blat) * 1e3
-    brhs = sum(r["nrhs"] for r in sample)
     out["baseline_fp64_one_at_a_time"] = {
-        "sample_requests": n_baseline, "solves_per_s": round(brhs / sum(service), 2),
+        "solves_per_s": round(nrhs_total / sum(service), 2),
         "p50_ms": round(float(np.percentile(blat, 50)), 2), "p99_ms": round(float(np.percentile(blat, 99)), 2),
-        "note": "FIFO replay of measured service times at the same arrival rate"}
+        "note": "FIFO replay of the same arrivals; per-request service = nrhs x the measured plain FP64 solve "
+                "time of its class (gauge, Wilson/clover, tol) at kappa 0.117"}
     out["speedup_solves_per_s"] = round(out["solves_per_s"] / out["baseline_fp64_one_at_a_time"]["solves_per_s"], 3)
     log(f"serving: {out['solves_per_s']} solves/s p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms; "
         f"fp64 one-at-a-time {out['baseline_fp64_one_at_a_time']['solves_per_s']} solves/s")
@@ -227,7 +231,18 @@ def config4_section(q):
                                   "reliable_updates": [s.reliable_updates for s in stsm],
                                   "max_true_residual": max(s.true_residual for s in stsm)}}
     out["speedup_solves_per_s"] = round(t64 / tm, 3)
-    log(f"config4: plain fp64 12 sequential {t64:.2f} s, batched mixed {tm:.2f} s, x{t64/tm:.2f}")
+    # beyond the configuration (which names FP32): the same batch with the FP16 +
+    # per-site-norm inner solve
+    q.tune_hop(g, 2, 12, 12)
+    sph = q.default_solve_params(tol=1e-10, precision=2, graph=1)
+    q.solve_batch_device(g, op, xs, srcs, sph)  # warm
+    stsh, th = timed(lambda: q.solve_batch_device(g, op, xs, srcs, sph))
+    out["extra_batched_mixed_fp16"] = {"seconds": round(th, 3), "solves_per_s": round(12 / th, 3),
+                                       "iterations": [s.iterations for s in stsh],
+                                       "max_true_residual": max(s.true_residual for s in stsh),
+                                       "speedup_solves_per_s": round(t64 / th, 3)}
+    log(f"config4: plain fp64 12 sequential {t64:.2f} s, batched mixed {tm:.2f} s, x{t64/tm:.2f}; "
+        f"fp16 inner {th:.2f} s, x{t64/th:.2f}")
     g.destroy()
     return out
 
@@ -346,8 +361,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=240,
                     help="requests in the Poisson serving stream (2000 for the full configuration; 0 skips)")
-    ap.add_argument("--serving-baseline", type=int, default=12,
-                    help="requests of the stream re-run as plain FP64 one-at-a-time solves")
     ap.add_argument("--kernels-12rhs", action="store_true",
                     help="profiler driver: copy kernel, then the 32^3x64 FP32 hop with 1 and 12 right-hand sides")
     ap.add_argument("--section", choices=["config1", "config3", "config4", "serving", "dslash_12rhs"],
@@ -373,7 +386,7 @@ def main():
         return 0
     if args.section:
         fn = {"config1": config1_gpu, "config3": config3_section, "config4": config4_section,
-              "serving": lambda q_: serving_section(q_, args.serving_requests, args.serving_baseline),
+              "serving": lambda q_: serving_section(q_, args.serving_requests),
               "dslash_12rhs": multi_rhs_section}[args.section]
         print(json.dumps({args.section: fn(q)}))
         return 0
@@ -479,7 +492,7 @@ def main():
         out["config3_clover_24x48"] = config3_section(q)
         out["config4_clover_32x64_12rhs"] = config4_section(q)
         if args.serving_requests > 0:
-            out["config5_serving"] = serving_section(q, args.serving_requests, args.serving_baseline)
+            out["config5_serving"] = serving_section(q, args.serving_requests)
 
     if not (args.quick or args.no_cpu):
         out["cpu_baseline"] = cpu_baseline()
