@@ -115,6 +115,27 @@ int qcuApplyM(qcuGauge g, const qcuOperator* op, void* out, const void* in, int 
  * -1 if dagger), computed in `precision` */
 int qcuApplyMhat(qcuGauge g, const qcuOperator* op, void* out, const void* in, int dagger, int precision);
 
+/* Full-lattice compositions of the reference's clover operator path (reference
+ * src/qcu_clover_dslash.cu), for a caller that builds its own solver from them.  With
+ * M = A - kappa D (A = 1 for c_sw = 0) and s = -1 if dagger:
+ *   QCU_COMP_AINV_D            out = kappa A^-1 D_s in      cloverDslashOneRound :990-1011
+ *   QCU_COMP_ONE_MINUS_AINV_D  out = in - kappa A^-1 D_s in newFullCloverDslashOneRound :1013-1042
+ *   QCU_COMP_AINV              out = A^-1 in                invertCloverDslash :902-918
+ *   QCU_COMP_A                 out = A in                   preCloverDslash :927-946
+ *   QCU_COMP_NORMAL            out = M' M in, M = 1 - kappa A^-1 D_+, M' = 1 - kappa A^-1 D_-
+ *                                                           MmV_one_round :1044-1049
+ * The reference's invertCloverDslash and preCloverDslash apply the clover in place on
+ * fermion_out; here they read `in` and write `out` (pass the same pointer for in place).
+ * A = 1 + kappa c_sw T; the reference's normalisation is kappa = c_sw = 1.  FP64. */
+enum {
+  QCU_COMP_AINV_D = 0,
+  QCU_COMP_ONE_MINUS_AINV_D = 1,
+  QCU_COMP_AINV = 2,
+  QCU_COMP_A = 3,
+  QCU_COMP_NORMAL = 4
+};
+int qcuApplyComposite(qcuGauge g, const qcuOperator* op, int kind, void* out, const void* in, int dagger);
+
 /* ---- solver ----------------------------------------------------------------- */
 void qcuDefaultSolveParams(qcuSolveParams* sp);
 /* parameters cg_inverter uses */

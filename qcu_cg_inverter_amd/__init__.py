@@ -125,6 +125,7 @@ PROTOTYPES = {
     "qcuHopBatch": (_int, [_vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _int, _int, _int, _int, _int]),
     "qcuApplyM": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int]),
     "qcuApplyMhat": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int, _int]),
+    "qcuApplyComposite": (_int, [_vp, ctypes.POINTER(Operator), _int, _vp, _vp, _int]),
     "qcuDefaultSolveParams": (None, [ctypes.POINTER(SolveParams)]),
     "qcuSetDefaultSolve": (_int, [ctypes.POINTER(SolveParams)]),
     "qcuSolve": (_int, [_vp, ctypes.POINTER(Operator), ctypes.POINTER(SolveParams), _vp, _vp,
@@ -304,6 +305,18 @@ def apply_m(g: Gauge, op: Operator, inp: np.ndarray, dagger=0):
     a = DeviceBuffer.from_numpy(np.ascontiguousarray(inp, np.complex128))
     o = DeviceBuffer(2 * vh * 192)
     check(lib().qcuApplyM(g.handle, ctypes.byref(op), o.ptr, a.ptr, dagger))
+    return o.to_numpy((2, vh, 4, 3))
+
+
+COMP_AINV_D, COMP_ONE_MINUS_AINV_D, COMP_AINV, COMP_A, COMP_NORMAL = range(5)
+
+
+def apply_composite(g: Gauge, op: Operator, kind: int, inp: np.ndarray, dagger=0):
+    """Full-lattice composition of the reference's clover path (qcuApplyComposite)."""
+    vh = g.vh
+    a = DeviceBuffer.from_numpy(np.ascontiguousarray(inp, np.complex128))
+    o = DeviceBuffer(2 * vh * 192)
+    check(lib().qcuApplyComposite(g.handle, ctypes.byref(op), kind, o.ptr, a.ptr, dagger))
     return o.to_numpy((2, vh, 4, 3))
 
 

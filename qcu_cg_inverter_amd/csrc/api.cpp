@@ -598,6 +598,15 @@ int qcuApplyM(qcuGauge h, const qcuOperator* op, void* out, const void* in, int 
   });
 }
 
+int qcuApplyComposite(qcuGauge h, const qcuOperator* op, int kind, void* out, const void* in, int dagger) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(h && op && out && in, "qcuApplyComposite: NULL argument");
+    apply_composite_iface(*reinterpret_cast<GaugeField*>(h), *op, kind, out, in, dagger ? 1 : 0);
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
 int qcuApplyMhat(qcuGauge h, const qcuOperator* op, void* out, const void* in, int dagger, int precision) {
   return guarded([&] {
     ensure_init();

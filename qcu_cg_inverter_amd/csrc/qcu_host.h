@@ -128,5 +128,6 @@ void apply_mhat_iface(GaugeField& g, const qcuOperator& op, void* out, const voi
 // forget solver state that refers to gauge g (all gauges if null)
 void drop_cached_batches(GaugeField* g);
 void apply_m_iface(GaugeField& g, const qcuOperator& op, void* out, const void* in, int dagger);
+void apply_composite_iface(GaugeField& g, const qcuOperator& op, int kind, void* out, const void* in, int dagger);
 
 }  // namespace qcu

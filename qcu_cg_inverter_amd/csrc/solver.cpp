@@ -175,6 +175,59 @@ void apply_mhat_iface(GaugeField& g, const qcuOperator& op, void* out, const voi
   QHIP(hipGetLastError());
 }
 
+// out_p = kappa A_pp^-1 D^s in_(1-p) for both parities (A = 1 for Wilson)
+static void ainv_d(OpView& o, int dag, const Vec in[2], const Vec out[2], hipStream_t st) {
+  for (int p = 0; p < 2; ++p) {
+    if (o.clover)
+      hop(*o.g, F64, o.op.recon, EPI_CLOV, p, dag, o.tbc(), in[1 - p], out[p], vec(nullptr), o.cf->ainv(F64, p),
+          o.op.kappa, 1, nullptr, st);
+    else
+      hop(*o.g, F64, o.op.recon, EPI_PLAIN, p, dag, o.tbc(), in[1 - p], out[p], vec(nullptr), nullptr, o.op.kappa, 1,
+          nullptr, st);
+  }
+}
+
+// out = in - kappa A^-1 D^s in (both parities; out must not alias in)
+static void one_minus_ainv_d(OpView& o, int dag, const Vec in[2], const Vec out[2], hipStream_t st) {
+  ainv_d(o, dag, in, out, st);
+  for (int p = 0; p < 2; ++p) launch_axpby(F64, 1.0, in[p], -1.0, out[p], o.g->geo.vh, 1, st);
+}
+
+void apply_composite_iface(GaugeField& g, const qcuOperator& op, int kind, void* out, const void* in, int dagger) {
+  QREQ(kind >= QCU_COMP_AINV_D && kind <= QCU_COMP_NORMAL, "qcuApplyComposite: unknown kind");
+  OpView o = make_view(g, op, F64);
+  const long vh = g.geo.vh;
+  const size_t fb = spinor_bytes(F64, vh);
+  Vec vi[2] = {vec(ctx().buf("cp_i0", fb)->p), vec(ctx().buf("cp_i1", fb)->p)};
+  Vec vo[2] = {vec(ctx().buf("cp_o0", fb)->p), vec(ctx().buf("cp_o1", fb)->p)};
+  hipStream_t st = stream();
+  for (int p = 0; p < 2; ++p)
+    launch_spinor_from_aos(F64, static_cast<const char*>(in) + p * vh * 192, 0, vi[p], vh, 1, st);
+  const int dag = dagger ? 1 : 0;
+  switch (kind) {
+    case QCU_COMP_AINV_D: ainv_d(o, dag, vi, vo, st); break;
+    case QCU_COMP_ONE_MINUS_AINV_D: one_minus_ainv_d(o, dag, vi, vo, st); break;
+    case QCU_COMP_AINV:
+    case QCU_COMP_A:
+      for (int p = 0; p < 2; ++p) {
+        if (o.clover)
+          launch_clover_apply(F64, kind == QCU_COMP_AINV ? o.cf->ainv(F64, p) : o.cf->a(F64, p), vi[p], vo[p], vh, 1,
+                              st);
+        else
+          QHIP(hipMemcpyAsync(vo[p].p, vi[p].p, fb, hipMemcpyDeviceToDevice, st));
+      }
+      break;
+    default: {  // QCU_COMP_NORMAL: M' M in
+      Vec vt[2] = {vec(ctx().buf("cp_t0", fb)->p), vec(ctx().buf("cp_t1", fb)->p)};
+      one_minus_ainv_d(o, 0, vi, vt, st);
+      one_minus_ainv_d(o, 1, vt, vo, st);
+    }
+  }
+  for (int p = 0; p < 2; ++p)
+    launch_spinor_to_aos(F64, vo[p], static_cast<char*>(out) + p * vh * 192, 0, vh, 1, st);
+  QHIP(hipGetLastError());
+}
+
 void apply_m_iface(GaugeField& g, const qcuOperator& op, void* out, const void* in, int dagger) {
   OpView o = make_view(g, op, F64);
   long vh = g.geo.vh;
