@@ -128,6 +128,8 @@ struct HopArgs {
   int ngroups;                           // blocks per site range (set by the launcher)
   int gmajor;                            // 1: right-hand side group is the slowest block index
   int rpw;                               // right-hand sides per wave (1, 2 or 4)
+  int spt;                               // site blocks walked by each block (sites per thread)
+  long nsb;                              // site blocks of the lattice (set by the launcher)
   int clov_shared;                       // every right-hand side uses clov[0]
   Walk walk;                             // site traversal order
 };

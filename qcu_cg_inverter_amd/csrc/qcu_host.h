@@ -67,7 +67,8 @@ struct GaugeField {
 // sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off, the
 // (y, z, t) tile of the site walk (tz = 0: storage order; ty = 0: whole xy-planes),
 // whether the right-hand side group is the slowest block index (gm) or the fastest,
-// and `rpw` right-hand sides per wave (lanes of a wave sharing sites)
+// `rpw` right-hand sides per wave (lanes of a wave sharing sites), and `spt` site
+// blocks each block walks (sites per thread)
 struct Tune {
   int block = 256;
   int rb = 1;
@@ -78,6 +79,7 @@ struct Tune {
   int gm = 0;
   int ty = 0;
   int rpw = 1;
+  int spt = 1;
 };
 
 struct Context {
