@@ -1,16 +1,19 @@
 // serve.cpp -- request loop with continuous batching over resident gauge fields.
 //
 // The reference has no serving path; this is the caller side of its solver.  One
-// worker thread owns the device work.  Each scheduling round it
-//   1. admits queued right-hand sides into batches (one CgBatch per gauge and
-//      operator type, each on its own stream) as slots and the HBM budget allow,
-//   2. launches one chunk of CG iterations on every batch that has work, so batches
-//      on different gauges overlap on the device,
-//   3. harvests: reliable updates, converged right-hand sides leave their slots, and
-//      requests whose right-hand sides are all done complete.
-// Empty batches are released when queued work needs their memory.  A right-hand side
-// enters at a chunk boundary and every decision about it uses only its own numbers,
-// so its result is the same whichever requests share its batch.
+// worker thread owns the device work and keeps every batch (one CgBatch per gauge and
+// operator type, each on its own stream) going on its own clock.  Each pass it
+//   1. harvests every batch whose chunk of CG iterations has finished: reliable
+//      updates, converged right-hand sides leave their slots, and requests whose
+//      right-hand sides are all done complete;
+//   2. admits queued right-hand sides, in arrival order, into idle batches (or new
+//      ones) as slots and the HBM budget allow;
+//   3. launches the next chunk on every idle batch that has work.
+// So a small lattice's batch is not held back by a large one's chunks, and batches
+// on different gauges overlap on the device.  Empty batches are released when queued
+// work needs their memory.  A right-hand side enters at a chunk boundary and every
+// decision about it uses only its own numbers, so its result is the same whichever
+// requests share its batch.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -44,6 +47,7 @@ struct BatchEntry {
   bool clover = false;
   int recon = 0, tbc = 0;
   size_t bytes = 0;
+  bool in_flight = false;  // a chunk is queued on the stream and not yet harvested
   std::map<uint64_t, std::pair<uint64_t, int>> slot_req;  // slot tag -> (request id, rhs)
 };
 
@@ -62,12 +66,13 @@ struct Server {
   std::string error;
 
   void run();
+  bool busy();
   bool admit_one(uint64_t id, Request& r);
   BatchEntry* find_batch(GaugeField* g, bool clover, int recon, int tbc);
   BatchEntry* new_batch(GaugeField* g, bool clover, int recon, int tbc);
   size_t clover_cost(GaugeField* g, double ck);
   void release_clover(GaugeField* g, double ck);
-  void round();
+  bool round();
 };
 
 size_t Server::clover_cost(GaugeField* g, double ck) {
@@ -104,7 +109,7 @@ BatchEntry* Server::new_batch(GaugeField* g, bool clover, int recon, int tbc) {
   if (used + need > budget) {
     // give back idle batches first
     for (size_t i = 0; i < batches.size();) {
-      if (batches[i]->batch->active() == 0) {
+      if (batches[i]->batch->active() == 0 && !batches[i]->in_flight) {
         used -= batches[i]->bytes;
         QHIP(hipStreamSynchronize(batches[i]->stream));
         QHIP(hipStreamDestroy(batches[i]->stream));
@@ -141,6 +146,7 @@ bool Server::admit_one(uint64_t id, Request& r) {
   const double ck = r.spec.kappa * r.spec.csw;
   while (r.admitted < r.spec.nrhs) {
     BatchEntry* be = find_batch(g, clover, r.spec.recon, r.spec.t_boundary);
+    if (be && be->in_flight) return false;  // slots open at its next chunk boundary
     if (clover) {
       size_t cc = clover_cost(g, ck);
       if (cc && used + cc > budget) return false;
@@ -175,38 +181,16 @@ bool Server::admit_one(uint64_t id, Request& r) {
   return true;
 }
 
-void Server::round() {
-  // 1. admission, in arrival order; a request that does not fit keeps its place
-  std::vector<uint64_t> ids;
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    ids.assign(queue.begin(), queue.end());
-  }
-  std::vector<uint64_t> admitted_all;
-  for (uint64_t id : ids) {
-    Request* r;
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      r = &reqs[id];
-    }
-    if (admit_one(id, *r)) admitted_all.push_back(id);
-    else break;  // keep arrival order for the memory budget
-  }
-  if (!admitted_all.empty()) {
-    std::lock_guard<std::mutex> lk(mu);
-    for (uint64_t id : admitted_all)
-      for (auto it = queue.begin(); it != queue.end(); ++it)
-        if (*it == id) {
-          queue.erase(it);
-          break;
-        }
-  }
-  // 2. one chunk on every busy batch, all in flight together
-  for (auto& b : batches)
-    if (b->batch->active() > 0) b->batch->launch_chunk();
-  // 3. harvest
+bool Server::round() {
+  bool progress = false;
+  // 1. harvest every batch whose chunk has finished
   for (auto& b : batches) {
-    if (b->batch->active() == 0) continue;
+    if (!b->in_flight) continue;
+    const hipError_t q = hipStreamQuery(b->stream);
+    if (q == hipErrorNotReady) continue;
+    QHIP(q);
+    b->in_flight = false;
+    progress = true;
     b->batch->harvest();
     for (const SlotResult& res : b->batch->take_finished()) {
       auto it = b->slot_req.find(res.tag);
@@ -227,20 +211,64 @@ void Server::round() {
       }
     }
   }
+  // 2. admission, in arrival order; a request that does not fit (or whose batch is
+  //    mid-chunk) keeps its place and the ones behind it wait
+  std::vector<uint64_t> ids;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    ids.assign(queue.begin(), queue.end());
+  }
+  std::vector<uint64_t> admitted_all;
+  for (uint64_t id : ids) {
+    Request* r;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      r = &reqs[id];
+    }
+    const int before = r->admitted;
+    const bool all_in = admit_one(id, *r);
+    progress |= r->admitted != before;
+    if (all_in) admitted_all.push_back(id);
+    else break;
+  }
+  if (!admitted_all.empty()) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (uint64_t id : admitted_all)
+      for (auto it = queue.begin(); it != queue.end(); ++it)
+        if (*it == id) {
+          queue.erase(it);
+          break;
+        }
+  }
+  // 3. the next chunk on every idle batch with work
+  for (auto& b : batches)
+    if (!b->in_flight && b->batch->active() > 0) {
+      b->batch->launch_chunk();
+      b->in_flight = true;
+      progress = true;
+    }
+  return progress;
+}
+
+bool Server::busy() {
+  for (auto& b : batches)
+    if (b->in_flight || b->batch->active() > 0) return true;
+  return false;
 }
 
 void Server::run() {
+  bool progress = true;
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(mu);
-      bool busy = false;
-      for (auto& b : batches) busy |= b->batch->active() > 0;
-      if (!busy) cv_work.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !queue.empty(); });
+      if (!busy()) cv_work.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !queue.empty(); });
       if (stop) break;
     }
+    // nothing finished and nothing admitted last pass: give the device a moment
+    if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
     try {
       std::lock_guard<std::recursive_mutex> g(ctx().mu);
-      round();
+      progress = round();
     } catch (const std::exception& e) {
       std::lock_guard<std::mutex> lk(mu);
       error = e.what();
