@@ -164,6 +164,13 @@ def serving_section(q, n_requests=240, seed=2024):
            "all_converged": all(s.converged for s in sts),
            "max_true_residual_over_tol": max(s.true_residual / r["tol"] for s, r in zip(sts, reqs)),
            "peak_device_bytes": usage["peak_bytes"]}
+    by_vol = {}
+    for vi, vol in enumerate(vols):
+        lv = np.array([l for l, r in zip(lat, reqs) if r["g"] // 2 == vi])
+        if lv.size:
+            by_vol["x".join(map(str, vol))] = {"requests": int(lv.size), "p50_ms": round(float(np.percentile(lv, 50)), 1),
+                                               "p99_ms": round(float(np.percentile(lv, 99)), 1)}
+    out["latency_by_volume"] = by_vol
     # baseline: plain FP64, one right-hand side at a time.  One solve is timed per
     # class (gauge, Wilson or clover, tol) at kappa = 0.117, then the whole stream is
     # replayed through a FIFO queue with the same arrival times, each request taking

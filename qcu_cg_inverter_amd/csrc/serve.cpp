@@ -18,6 +18,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <thread>
 
@@ -67,7 +68,9 @@ struct Server {
 
   void run();
   bool busy();
-  bool admit_one(uint64_t id, Request& r);
+  enum Admit { ADMIT_ALL, ADMIT_WAIT, ADMIT_NO_MEMORY };
+  Admit admit_one(uint64_t id, Request& r);
+  void complete_rhs(Request& r, bool converged, int iterations, double true_residual);
   BatchEntry* find_batch(GaugeField* g, bool clover, int recon, int tbc);
   BatchEntry* new_batch(GaugeField* g, bool clover, int recon, int tbc);
   size_t clover_cost(GaugeField* g, double ck);
@@ -140,19 +143,33 @@ BatchEntry* Server::new_batch(GaugeField* g, bool clover, int recon, int tbc) {
 }
 
 // move as many right-hand sides of request id into batches as fit; true when all are in
-bool Server::admit_one(uint64_t id, Request& r) {
+// a right-hand side of r is done (solved, or rejected at admission); the request
+// completes with its last one.  Caller holds mu.
+void Server::complete_rhs(Request& r, bool converged, int iterations, double true_residual) {
+  r.finished++;
+  r.all_converged &= converged;
+  r.max_iter_seen = std::max(r.max_iter_seen, iterations);
+  r.worst_res = std::max(r.worst_res, true_residual);
+  if (r.finished == r.spec.nrhs) {
+    r.done = true;
+    r.done_at = Clock::now();
+    cv_done.notify_all();
+  }
+}
+
+Server::Admit Server::admit_one(uint64_t id, Request& r) {
   GaugeField* g = reinterpret_cast<GaugeField*>(r.spec.gauge);
   const bool clover = r.spec.csw != 0.0;
   const double ck = r.spec.kappa * r.spec.csw;
   while (r.admitted < r.spec.nrhs) {
     BatchEntry* be = find_batch(g, clover, r.spec.recon, r.spec.t_boundary);
-    if (be && be->in_flight) return false;  // slots open at its next chunk boundary
+    if (be && be->in_flight) return ADMIT_WAIT;  // slots open at its next chunk boundary
     if (clover) {
       size_t cc = clover_cost(g, ck);
-      if (cc && used + cc > budget) return false;
+      if (cc && used + cc > budget) return ADMIT_NO_MEMORY;
     }
     if (!be) be = new_batch(g, clover, r.spec.recon, r.spec.t_boundary);
-    if (!be) return false;
+    if (!be) return ADMIT_NO_MEMORY;
     if (clover) {
       size_t cc = clover_cost(g, ck);
       used += cc;
@@ -169,16 +186,26 @@ bool Server::admit_one(uint64_t id, Request& r) {
     s.tol = r.spec.tol;
     s.max_iter = r.spec.max_iter;
     s.tag = next_tag++;
-    be->slot_req[s.tag] = {id, k};
-    be->batch->add(s);  // device work: the request lock is not held here
+    bool rejected = false;
+    try {
+      be->batch->add(s);  // device work: the request lock is not held here
+      be->slot_req[s.tag] = {id, k};
+    } catch (const Error& e) {
+      // this right-hand side cannot be solved (e.g. a non-finite source): it fails
+      // alone, the server and the other requests go on
+      if (e.code == QCU_ERROR_HIP || e.code == QCU_ERROR_OUT_OF_MEMORY) throw;
+      if (clover) release_clover(g, ck);
+      rejected = true;
+    }
     std::lock_guard<std::mutex> lk(mu);
     if (!r.started) {
       r.started = true;
       r.first_admit = Clock::now();
     }
     r.admitted++;
+    if (rejected) complete_rhs(r, false, 0, std::numeric_limits<double>::infinity());
   }
-  return true;
+  return ADMIT_ALL;
 }
 
 bool Server::round() {
@@ -199,20 +226,13 @@ bool Server::round() {
       b->slot_req.erase(it);
       std::lock_guard<std::mutex> lk(mu);
       Request& r = reqs[id];
-      r.finished++;
-      r.all_converged &= res.stats.converged != 0;
-      r.max_iter_seen = std::max(r.max_iter_seen, res.stats.iterations);
-      r.worst_res = std::max(r.worst_res, res.stats.true_residual);
       if (r.spec.csw != 0.0) release_clover(b->gauge, r.spec.kappa * r.spec.csw);
-      if (r.finished == r.spec.nrhs) {
-        r.done = true;
-        r.done_at = Clock::now();
-        cv_done.notify_all();
-      }
+      complete_rhs(r, res.stats.converged != 0, res.stats.iterations, res.stats.true_residual);
     }
   }
-  // 2. admission, in arrival order; a request that does not fit (or whose batch is
-  //    mid-chunk) keeps its place and the ones behind it wait
+  // 2. admission in arrival order.  A request whose batch is mid-chunk keeps its place
+  //    while later ones (other gauges) go ahead; one that does not fit the memory
+  //    budget keeps its place and the ones behind it wait
   std::vector<uint64_t> ids;
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -226,10 +246,10 @@ bool Server::round() {
       r = &reqs[id];
     }
     const int before = r->admitted;
-    const bool all_in = admit_one(id, *r);
+    const Admit a = admit_one(id, *r);
     progress |= r->admitted != before;
-    if (all_in) admitted_all.push_back(id);
-    else break;
+    if (a == ADMIT_ALL) admitted_all.push_back(id);
+    else if (a == ADMIT_NO_MEMORY) break;
   }
   if (!admitted_all.empty()) {
     std::lock_guard<std::mutex> lk(mu);

@@ -103,3 +103,26 @@ def test_server_rejects_over_budget(q):
         s.submit(g, 0.12, 0.0, 1e-8, 1, seed=1)
     assert e.value.code == 8
     s.destroy()
+
+
+def test_server_bad_request_fails_alone(q):
+    """A request with a non-finite source fails on its own (not converged, infinite
+    residual); requests before and after it on the same gauge still converge."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=44, eps=0.25)
+    vh = q.half_volume(L)
+    bad = orc.source_gauss(L, 9)
+    bad[1, 5, 2, 1] = np.nan
+    bbad = q.DeviceBuffer.from_numpy(bad)
+    xbad = q.DeviceBuffer(2 * vh * 192)
+    s = q.Server(precision=1, max_batch=4, chunk=4)
+    before = s.submit(g, 0.12, 1.0, 1e-10, 2, seed=11)
+    rid = s.submit(g, 0.12, 1.0, 1e-10, 1, xs=[xbad], bs=[bbad])
+    after = s.submit(g, 0.12, 1.0, 1e-10, 3, seed=12)
+    s.drain()
+    st = s.poll(rid)
+    assert st.done and not st.converged and not np.isfinite(st.true_residual)
+    for other in (before, after):
+        so = s.poll(other)
+        assert so.done and so.converged and so.true_residual <= 1e-10
+    s.destroy()
