@@ -17,8 +17,6 @@ __device__ __forceinline__ double fmx(double a, double b, double c) { return __b
 __device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 template <typename T>
-__device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
-template <typename T>
 __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
 template <typename T>
 __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }

@@ -69,7 +69,7 @@ struct Server {
   void run();
   bool busy();
   enum Admit { ADMIT_ALL, ADMIT_WAIT, ADMIT_NO_MEMORY };
-  Admit admit_one(uint64_t id, Request& r);
+  Admit admit_one(uint64_t id, Request& r, bool allow_alloc);
   void complete_rhs(Request& r, bool converged, int iterations, double true_residual);
   BatchEntry* find_batch(GaugeField* g, bool clover, int recon, int tbc);
   BatchEntry* new_batch(GaugeField* g, bool clover, int recon, int tbc);
@@ -157,7 +157,8 @@ void Server::complete_rhs(Request& r, bool converged, int iterations, double tru
   }
 }
 
-Server::Admit Server::admit_one(uint64_t id, Request& r) {
+// allow_alloc = false: only free slots of existing batches and existing clover fields
+Server::Admit Server::admit_one(uint64_t id, Request& r, bool allow_alloc) {
   GaugeField* g = reinterpret_cast<GaugeField*>(r.spec.gauge);
   const bool clover = r.spec.csw != 0.0;
   const double ck = r.spec.kappa * r.spec.csw;
@@ -166,9 +167,9 @@ Server::Admit Server::admit_one(uint64_t id, Request& r) {
     if (be && be->in_flight) return ADMIT_WAIT;  // slots open at its next chunk boundary
     if (clover) {
       size_t cc = clover_cost(g, ck);
-      if (cc && used + cc > budget) return ADMIT_NO_MEMORY;
+      if (cc && (!allow_alloc || used + cc > budget)) return ADMIT_NO_MEMORY;
     }
-    if (!be) be = new_batch(g, clover, r.spec.recon, r.spec.t_boundary);
+    if (!be && allow_alloc) be = new_batch(g, clover, r.spec.recon, r.spec.t_boundary);
     if (!be) return ADMIT_NO_MEMORY;
     if (clover) {
       size_t cc = clover_cost(g, ck);
@@ -231,14 +232,16 @@ bool Server::round() {
     }
   }
   // 2. admission in arrival order.  A request whose batch is mid-chunk keeps its place
-  //    while later ones (other gauges) go ahead; one that does not fit the memory
-  //    budget keeps its place and the ones behind it wait
+  //    while later ones go ahead.  Once one does not fit the memory budget, the ones
+  //    behind it may still fill free slots of existing batches, but nothing new is
+  //    allocated until it is in, so it is not starved
   std::vector<uint64_t> ids;
   {
     std::lock_guard<std::mutex> lk(mu);
     ids.assign(queue.begin(), queue.end());
   }
   std::vector<uint64_t> admitted_all;
+  bool allow_alloc = true;
   for (uint64_t id : ids) {
     Request* r;
     {
@@ -246,10 +249,10 @@ bool Server::round() {
       r = &reqs[id];
     }
     const int before = r->admitted;
-    const Admit a = admit_one(id, *r);
+    const Admit a = admit_one(id, *r, allow_alloc);
     progress |= r->admitted != before;
     if (a == ADMIT_ALL) admitted_all.push_back(id);
-    else if (a == ADMIT_NO_MEMORY) break;
+    else if (a == ADMIT_NO_MEMORY) allow_alloc = false;
   }
   if (!admitted_all.empty()) {
     std::lock_guard<std::mutex> lk(mu);
