@@ -179,7 +179,9 @@ typedef struct {
   int t_boundary, recon;
   int nrhs;                /* 1..16 */
   void* const* x;          /* nrhs solution buffers (full spinors) or NULL: residuals only */
-  const void* const* b;    /* nrhs sources or NULL: Gaussian sources, seeds seed + r */
+  const void* const* b;    /* nrhs sources (read again when each right-hand side finishes,
+                              so unchanged until the request is done) or NULL: Gaussian
+                              sources, seeds seed + r */
   uint64_t seed;
   int max_iter;            /* <= 0: 10 * volume */
 } qcuRequest;

@@ -10,7 +10,7 @@ namespace qcu {
 
 size_t CgBatch::bytes_for(long vh, int prec, int cap) {
   size_t fb = spinor_bytes(F64, vh), pb = spinor_bytes(prec, vh), xb = spinor_bytes(F32, vh);
-  size_t s = 6 * fb * cap + (prec == F64 ? 4 : 5) * pb * cap + (prec == F64 ? 0 : xb * cap);
+  size_t s = 4 * fb * cap + (prec == F64 ? 4 : 5) * pb * cap + (prec == F64 ? 0 : xb * cap);
   s += sizeof(double) * (4 * n_partials(vh) * cap + cap) + sizeof(CgScal) * cap + 2 * (size_t)vh * 192;
   return s;
 }
@@ -30,14 +30,14 @@ CgBatch::CgBatch(GaugeField& g, int prec, int recon, int t_boundary, bool clover
   fb_ = (long)spinor_bytes(F64, vh_);
   pb_ = (long)spinor_bytes(prec, vh_);
   xb_ = (long)spinor_bytes(xprec_, vh_);
-  f64_.alloc((size_t)6 * fb_ * cap_);
+  // the source itself is not kept: it is read again (or regenerated from its seed)
+  // when the slot finishes
+  f64_.alloc((size_t)4 * fb_ * cap_);
   char* f = f64_.c();
-  be_ = vec(f, fb_);
-  bo_ = vec(f + 1 * fb_ * cap_, fb_);
-  bh_ = vec(f + 2 * fb_ * cap_, fb_);
-  xe_ = vec(f + 3 * fb_ * cap_, fb_);
-  shi_ = vec(f + 4 * fb_ * cap_, fb_);
-  t64_ = vec(f + 5 * fb_ * cap_, fb_);
+  bh_ = vec(f, fb_);
+  xe_ = vec(f + 1 * fb_ * cap_, fb_);
+  shi_ = vec(f + 2 * fb_ * cap_, fb_);
+  t64_ = vec(f + 3 * fb_ * cap_, fb_);
   int nslo = prec == F64 ? 4 : 5;
   slo_.alloc((size_t)nslo * pb_ * cap_);
   char* s = slo_.c();
@@ -88,7 +88,7 @@ void CgBatch::check_operator(const SlotSpec& s) const {
 }
 
 Vec CgBatch::at(Vec v, int r) const {
-  const Vec base[] = {be_, bo_, bh_, xe_, shi_, t64_, S_, R_, P_, T_, TO_, X_};
+  const Vec base[] = {bh_, xe_, shi_, t64_, S_, R_, P_, T_, TO_, X_};
   bool ok = false;
   for (const Vec& b : base) ok |= b.p == v.p && b.stride == v.stride;
   if (!ok || r < 0 || r >= cap_) throw Error(QCU_ERROR_STATE, "internal: batch slot view out of range");
@@ -139,19 +139,15 @@ void CgBatch::add(const SlotSpec& spec) {
     QHIP(hipStreamSynchronize(ctx().stream));
   }
   if (sl.spec.max_iter <= 0) sl.spec.max_iter = (int)std::min<long>(20 * vh_, 1000000);
-  const void* b = spec.b;
-  if (!b) {
-    aos_.ensure((size_t)2 * vh_ * 192);
-    launch_source_gauss(aos_.p, 2 * vh_ * 12, spec.seed, st_);
-    b = aos_.p;
-  }
-  launch_spinor_from_aos(F64, b, 0, at(be_, r), vh_, 1, st_);
-  launch_spinor_from_aos(F64, static_cast<const char*>(b) + vh_ * 192, 0, at(bo_, r), vh_, 1, st_);
-  QHIP(hipGetLastError());
   slots_.push_back(sl);
   n_++;
   Slot& s = slots_[r];
-  s.bnorm2 = norm2_slot(be_, r) + norm2_slot(bo_, r);
+  // b_e in shi_, b_o in t64_ (both scratch until the iteration starts)
+  const char* b = static_cast<const char*>(source_aos(s));
+  launch_spinor_from_aos(F64, b, 0, at(shi_, r), vh_, 1, st_);
+  launch_spinor_from_aos(F64, b + vh_ * 192, 0, at(t64_, r), vh_, 1, st_);
+  QHIP(hipGetLastError());
+  s.bnorm2 = norm2_slot(shi_, r) + norm2_slot(t64_, r);
   if (!std::isfinite(s.bnorm2)) {
     slots_.pop_back();
     n_--;
@@ -159,14 +155,9 @@ void CgBatch::add(const SlotSpec& spec) {
   }
   // b^ = b_e + kappa D_eo A_oo^-1 b_o
   const double kap = spec.kappa;
-  if (clover_) {
-    launch_clover_apply(F64, s.cf->ainv(F64, 1), at(bo_, r), at(t64_, r), vh_, 1, st_);
-    hop(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(t64_, r), at(bh_, r), at(be_, r), nullptr, kap, 1, nullptr,
-        st_);
-  } else {
-    hop(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(bo_, r), at(bh_, r), at(be_, r), nullptr, kap, 1, nullptr,
-        st_);
-  }
+  if (clover_) launch_clover_apply(F64, s.cf->ainv(F64, 1), at(t64_, r), at(t64_, r), vh_, 1, st_);  // in place
+  hop(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(t64_, r), at(bh_, r), at(shi_, r), nullptr, kap, 1, nullptr,
+      st_);
   launch_zero(F64, at(xe_, r), vh_, 1, st_);
   QHIP(hipMemcpyAsync(at(shi_, r).p, at(bh_, r).p, fb_, hipMemcpyDeviceToDevice, st_));
   if (prec_ != F64) {
@@ -318,7 +309,17 @@ std::vector<double> CgBatch::reliable_updates(int lo, int hi, const std::vector<
   return out;
 }
 
-// x_o, true residual, solution output and stats for slot r, then compaction
+// the slot's interface-layout source: the caller's buffer, or regenerated from its
+// seed into the batch's staging buffer
+const void* CgBatch::source_aos(const Slot& sl) {
+  if (sl.spec.b) return sl.spec.b;
+  aos_.ensure((size_t)2 * vh_ * 192);
+  launch_source_gauss(aos_.p, 2 * vh_ * 12, sl.spec.seed, st_);
+  return aos_.p;
+}
+
+// x_o, true residual, solution output and stats for slot r, then compaction.  Uses
+// the slot's bh_, shi_ and t64_ as scratch (b^ is no longer needed).
 void CgBatch::finish(int r, bool breakdown) {
   Slot& sl = slots_[r];
   const double kap = sl.spec.kappa;
@@ -326,28 +327,31 @@ void CgBatch::finish(int r, bool breakdown) {
     launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
     launch_zero(xprec_, at(X_, r), vh_, 1, st_);
   }
-  Vec xo = at(shi_, r);
+  const char* b = static_cast<const char*>(source_aos(sl));
+  Vec xo = at(shi_, r), tmp = at(bh_, r), bq = at(t64_, r);
+  launch_spinor_from_aos(F64, b + vh_ * 192, 0, bq, vh_, 1, st_);  // b_o
+  // x_o = A_oo^-1 (b_o + kappa D_oe x_e)
   if (clover_) {
-    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), at(t64_, r), at(bo_, r), nullptr, kap, 1, nullptr,
-        st_);
-    launch_clover_apply(F64, sl.cf->ainv(F64, 1), at(t64_, r), xo, vh_, 1, st_);
+    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), tmp, bq, nullptr, kap, 1, nullptr, st_);
+    launch_clover_apply(F64, sl.cf->ainv(F64, 1), tmp, xo, vh_, 1, st_);
   } else {
-    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), xo, at(bo_, r), nullptr, kap, 1, nullptr, st_);
+    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), xo, bq, nullptr, kap, 1, nullptr, st_);
   }
-  // r = b - M x with M = A - kappa D on both parities
-  Vec re = at(bh_, r), ro = at(t64_, r);
-  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 0, 0, (double)tbc_, xo, re, at(xe_, r),
-      clover_ ? sl.cf->a(F64, 0) : nullptr, -kap, 1, nullptr, st_);
-  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), ro, xo,
+  // r = b - M x with M = A - kappa D, odd then even parity
+  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), tmp, xo,
       clover_ ? sl.cf->a(F64, 1) : nullptr, -kap, 1, nullptr, st_);
-  launch_axpby(F64, 1.0, at(be_, r), -1.0, re, vh_, 1, st_);
-  launch_axpby(F64, 1.0, at(bo_, r), -1.0, ro, vh_, 1, st_);
+  launch_axpby(F64, 1.0, bq, -1.0, tmp, vh_, 1, st_);
+  double nr = norm2_slot(bh_, r);  // |r_o|^2
+  launch_spinor_from_aos(F64, b, 0, bq, vh_, 1, st_);  // b_e
+  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 0, 0, (double)tbc_, xo, tmp, at(xe_, r),
+      clover_ ? sl.cf->a(F64, 0) : nullptr, -kap, 1, nullptr, st_);
+  launch_axpby(F64, 1.0, bq, -1.0, tmp, vh_, 1, st_);
   if (sl.spec.x) {
     launch_spinor_to_aos(F64, at(xe_, r), sl.spec.x, 0, vh_, 1, st_);
     launch_spinor_to_aos(F64, xo, static_cast<char*>(sl.spec.x) + vh_ * 192, 0, vh_, 1, st_);
   }
   QHIP(hipGetLastError());
-  double nr = norm2_slot(bh_, r) + norm2_slot(t64_, r);  // re, ro
+  nr = norm2_slot(bh_, r) + nr;  // |r_e|^2 + |r_o|^2
   SlotResult res;
   res.tag = sl.spec.tag;
   res.stats.iterations = hsc_[r].iter;
@@ -365,7 +369,7 @@ void CgBatch::finish(int r, bool breakdown) {
 }
 
 void CgBatch::move_slot(int from, int to) {
-  Vec keep[] = {be_, bo_, bh_, xe_, shi_, P_};
+  Vec keep[] = {bh_, xe_, shi_, P_};
   for (const Vec& v : keep)
     QHIP(hipMemcpyAsync(at(v, to).p, at(v, from).p, v.stride, hipMemcpyDeviceToDevice, st_));
   if (prec_ != F64) {

@@ -67,6 +67,7 @@ class CgBatch {
   double reliable_update(int r);
   std::vector<double> reliable_updates(int lo, int hi, const std::vector<char>& upd);
   void finish(int r, bool breakdown);
+  const void* source_aos(const Slot& sl);
   void move_slot(int from, int to);
   // slot r of a batch field; v must be the field's base (slot 0), never a slot view
   Vec at(Vec v, int r) const;
@@ -81,7 +82,7 @@ class CgBatch {
   hipStream_t st_;
   long vh_, nw_, fb_, pb_, xb_;
   DevBuf f64_, slo_, xlo_, part_, scal_, tmp_, aos_;
-  Vec be_, bo_, bh_, xe_, shi_, t64_, S_, R_, P_, T_, TO_, X_;
+  Vec bh_, xe_, shi_, t64_, S_, R_, P_, T_, TO_, X_;
   double *part_rr_, *part_tt_, *part_ss_, *sums_;
   CgScal* dsc_;
   CgScal* hsc_;

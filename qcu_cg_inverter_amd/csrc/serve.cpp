@@ -128,7 +128,11 @@ BatchEntry* Server::new_batch(GaugeField* g, bool clover, int recon, int tbc) {
   qcuSolveParams sp;
   qcuDefaultSolveParams(&sp);
   sp.precision = cfg.precision;
-  sp.chunk = cfg.chunk;
+  // Chunks of about the same device time whatever the volume: streams share hardware
+  // queues, so a small lattice's chunk can wait behind a large one's already queued
+  // chunk.  cfg.chunk iterations up to a 65536-site half lattice, proportionally fewer
+  // (at least one) above.
+  sp.chunk = (int)std::max<long>(1, std::min<long>(cfg.chunk, (long)cfg.chunk * 65536 / g->geo.vh));
   sp.graph = cfg.graph;
   e->batch.reset(new CgBatch(*g, cfg.precision, recon, tbc, clover, cfg.max_batch, sp, e->stream));
   e->gauge = g;
