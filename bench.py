@@ -415,7 +415,7 @@ def main():
             key = f"hop:1:12:{q.half_volume(gk.L)}:12"
             with open(cache) as f:
                 rows = [ln.split() for ln in f if ln.startswith(key + " ")]
-            sh = dict(zip(q.SHAPE_KEYS, map(int, rows[-1][1:11])))
+            sh = dict(zip(q.SHAPE_KEYS, map(int, rows[-1][1:12])))
             if sh["rb"] * sh["rpw"] * sh["rpt"] < 12:
                 sh["group_major"] ^= 1
                 q.set_hop_shape(gk, 1, 12, 12, sh)

@@ -273,6 +273,7 @@ void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, 
   a.gmajor = t.gm;
   a.rpw = t.rpw;
   a.spt = t.spt;
+  a.cached = t.cached;
   a.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }

@@ -129,6 +129,7 @@ struct HopArgs {
   int gmajor;                            // 1: right-hand side group is the slowest block index
   int rpw;                               // right-hand sides per wave (1, 2 or 4)
   int spt;                               // site blocks walked by each block (sites per thread)
+  int cached;                            // 1: ordinary loads of links / clover / x, 0: streaming
   long nsb;                              // site blocks of the lattice (set by the launcher)
   int clov_shared;                       // every right-hand side uses clov[0]
   Walk walk;                             // site traversal order

@@ -82,6 +82,15 @@ __device__ __forceinline__ uint2 ld_nt(const uint2* p) {
   return uint2{v.x, v.y};
 }
 __device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return __builtin_nontemporal_load(p); }
+// NT = 1: nontemporal (streaming) load, which keeps data read once out of the caches'
+// way; NT = 0: ordinary load, which lets a field that fits the 256 MB Infinity Cache
+// be served from there on the next launch.  Which is faster depends on the lattice, so
+// the hop's launch shape carries the choice.
+template <int NT, typename V>
+__device__ __forceinline__ V ldp(const V* p) {
+  if (NT) return ld_nt(p);
+  return *p;
+}
 
 template <int P>
 struct Real { using T = float; };
@@ -111,11 +120,12 @@ struct SpinorIO<F64> {
       v[c] = {q.x, q.y};
     }
   }
+  template <int NT = 1>
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const double2* b = static_cast<const double2*>(base) + i + (long)(6 * h) * vh;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      double2 q = ld_nt(b + c * vh);
+      double2 q = ldp<NT>(b + c * vh);
       v[c] = {q.x, q.y};
     }
   }
@@ -141,6 +151,7 @@ struct SpinorIO<F32> {
   static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
     load_cached(base, vh, i, v);
   }
+  template <int NT = 1>  // FP32 spinors are always loaded cached
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const float4* b = static_cast<const float4*>(base) + i + (long)(3 * h) * vh;
 #pragma unroll
@@ -197,6 +208,7 @@ struct SpinorIO<F16> {
     load_cached(base, vh, i, v);
   }
   // components 0..5 are word 0 and the first half of word 1; 6..11 the rest
+  template <int NT = 1>  // FP16 spinors are always loaded cached
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const uint4* b = static_cast<const uint4*>(base) + i + (long)h * vh;
     float nrm = norms(base, vh)[i];
@@ -246,11 +258,12 @@ struct LinkIO<F64, R> {
   using T = double;
   static constexpr int NE = R == 12 ? 6 : 9;
   // plane base of (mu, parity)
+  template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     const double2* b = static_cast<const double2*>(g) + (long)(mu * 2 + par) * NE * vh + i;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      double2 q = ld_nt(b + e * vh);
+      double2 q = ldp<NT>(b + e * vh);
       u[e] = {q.x, q.y};
     }
     if (R == 12) recon_row2(u);
@@ -260,12 +273,13 @@ struct LinkIO<F64, R> {
 template <int R>
 struct LinkIO<F32, R> {
   using T = float;
+  template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     if (R == 12) {
       const float4* b = static_cast<const float4*>(g) + (long)(mu * 2 + par) * 3 * vh + i;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        float4 q = ld_nt(b + k * vh);
+        float4 q = ldp<NT>(b + k * vh);
         u[2 * k] = {q.x, q.y};
         u[2 * k + 1] = {q.z, q.w};
       }
@@ -274,7 +288,7 @@ struct LinkIO<F32, R> {
       const float2* b = static_cast<const float2*>(g) + (long)(mu * 2 + par) * 9 * vh + i;
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
-        float2 q = ld_nt(b + e * vh);
+        float2 q = ldp<NT>(b + e * vh);
         u[e] = {q.x, q.y};
       }
     }
@@ -284,12 +298,13 @@ struct LinkIO<F32, R> {
 template <int R>
 struct LinkIO<F16, R> {
   using T = float;
+  template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     if (R == 12) {
       const uint2* b = static_cast<const uint2*>(g) + (long)(mu * 2 + par) * 3 * vh + i;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        uint2 q = ld_nt(b + k * vh);
+        uint2 q = ldp<NT>(b + k * vh);
         u[2 * k] = {h2f(q.x & 0xffff), h2f(q.x >> 16)};
         u[2 * k + 1] = {h2f(q.y & 0xffff), h2f(q.y >> 16)};
       }
@@ -298,7 +313,7 @@ struct LinkIO<F16, R> {
       const unsigned* b = static_cast<const unsigned*>(g) + (long)(mu * 2 + par) * 9 * vh + i;
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
-        unsigned q = ld_nt(b + e * vh);
+        unsigned q = ldp<NT>(b + e * vh);
         u[e] = {h2f(q & 0xffff), h2f(q >> 16)};
       }
     }
@@ -324,11 +339,12 @@ struct CloverIO<F64> {
     }
   }
   // chiral block b (36 reals) only
+  template <int NT = 1>
   static __device__ __forceinline__ void load_block(const void* c, long vh, long i, int b, T m[36]) {
     const double2* p = static_cast<const double2*>(c) + i + (long)(18 * b) * vh;
 #pragma unroll
     for (int k = 0; k < 18; ++k) {
-      double2 q = ld_nt(p + k * vh);
+      double2 q = ldp<NT>(p + k * vh);
       m[2 * k] = q.x;
       m[2 * k + 1] = q.y;
     }
@@ -348,11 +364,12 @@ struct CloverIO<F32> {
       a[4 * k + 3] = q.w;
     }
   }
+  template <int NT = 1>
   static __device__ __forceinline__ void load_block(const void* c, long vh, long i, int b, T m[36]) {
     const float4* p = static_cast<const float4*>(c) + i + (long)(9 * b) * vh;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      float4 q = ld_nt(p + k * vh);
+      float4 q = ldp<NT>(p + k * vh);
       m[4 * k] = q.x;
       m[4 * k + 1] = q.y;
       m[4 * k + 2] = q.z;
@@ -369,11 +386,12 @@ struct CloverIO<F16> {
     for (int k = 0; k < 9; ++k) unpack8(ld_nt(b + k * vh), a + 8 * k);
   }
   // block 0 is halves 0..35 (words 0..4, first half of word 4), block 1 is 36..71
+  template <int NT = 1>
   static __device__ __forceinline__ void load_block(const void* c, long vh, long i, int b, T m[36]) {
     const uint4* p = static_cast<const uint4*>(c) + i + (long)(4 * b) * vh;
     float f[40];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) unpack8(ld_nt(p + k * vh), f + 8 * k);
+    for (int k = 0; k < 5; ++k) unpack8(ldp<NT>(p + k * vh), f + 8 * k);
 #pragma unroll
     for (int k = 0; k < 36; ++k) m[k] = b ? f[k + 4] : f[k];
   }

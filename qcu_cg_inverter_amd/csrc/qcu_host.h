@@ -67,8 +67,9 @@ struct GaugeField {
 // sites, `rpt` right-hand sides per thread, XCD-contiguous block order on/off, the
 // (y, z, t) tile of the site walk (tz = 0: storage order; ty = 0: whole xy-planes),
 // whether the right-hand side group is the slowest block index (gm) or the fastest,
-// `rpw` right-hand sides per wave (lanes of a wave sharing sites), and `spt` site
-// blocks each block walks (sites per thread)
+// `rpw` right-hand sides per wave (lanes of a wave sharing sites), `spt` site blocks
+// each block walks (sites per thread), and whether links / clover / x use ordinary
+// (cached) loads instead of streaming ones
 struct Tune {
   int block = 256;
   int rb = 1;
@@ -80,6 +81,7 @@ struct Tune {
   int ty = 0;
   int rpw = 1;
   int spt = 1;
+  int cached = 0;
 };
 
 struct Context {

@@ -34,8 +34,8 @@ void load_tune_cache() {
   while (fgets(line, sizeof line, f)) {
     Tune t;
     double ms;
-    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz, &t.tt,
-               &t.gm, &t.ty, &t.rpw, &t.spt, &ms) == 12)
+    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz,
+               &t.tt, &t.gm, &t.ty, &t.rpw, &t.spt, &t.cached, &ms) == 13)
       ctx().tune[key] = t;
   }
   fclose(f);
@@ -46,8 +46,8 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm,
-          t.ty, t.rpw, t.spt, ms);
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt,
+          t.gm, t.ty, t.rpw, t.spt, t.cached, ms);
   fclose(f);
 }
 
@@ -125,6 +125,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
     args.gmajor = t.gm;
     args.rpw = t.rpw;
     args.spt = t.spt;
+    args.cached = t.cached;
     args.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
     for (int w = 0; w < 2; ++w) launch_hop(prec, recon, epi, args, t.block, t.rb, t.rpt, stream());
     QHIP(hipEventRecord(e0, stream()));
@@ -147,12 +148,14 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   }
   auto by_time = [](const std::pair<double, Tune>& x, const std::pair<double, Tune>& y) { return x.first < y.first; };
   std::stable_sort(timed.begin(), timed.end(), by_time);
-  // ... then the fastest few walking 2 or 4 site blocks per block (sites per thread)
+  // ... then the fastest few walking 2 or 4 site blocks per block (sites per thread),
+  // and with ordinary instead of streaming loads
   const size_t top = timed.size() < 8 ? timed.size() : 8;
   for (size_t k = 0; k < top; ++k)
-    for (int spt = 2; spt <= 4; spt *= 2) {
+    for (int v = 0; v < 3; ++v) {
       Tune t = timed[k].second;
-      t.spt = spt;
+      if (v < 2) t.spt = 2 << v;
+      else t.cached = 1;
       timed.push_back({time_shape(t, EPI_PLAIN), t});
     }
   std::stable_sort(timed.begin(), timed.end(), by_time);
@@ -214,6 +217,7 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* s
       shape[7] = t.ty;
       shape[8] = t.rpw;
       shape[9] = t.spt;
+      shape[10] = t.cached;
     }
     return QCU_SUCCESS;
   } catch (const Error& e) {
@@ -242,9 +246,11 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     t.ty = shape[7];
     t.rpw = shape[8];
     t.spt = shape[9];
+    t.cached = shape[10];
     QREQ(t.block == 64 || t.block == 128 || t.block == 192 || t.block == 256, "block must be 64, 128, 192 or 256");
     QREQ(t.rpw == 1 || t.rpw == 2 || t.rpw == 4, "right-hand sides per wave must be 1, 2 or 4");
     QREQ(t.spt >= 1 && t.spt <= 64, "site blocks per block must be 1..64");
+    QREQ(t.cached == 0 || t.cached == 1, "load policy must be 0 (streaming) or 1 (cached)");
     QREQ(t.rb >= 1 && t.block % (64 * t.rb) == 0, "rb must divide block into 64-thread groups");
     QREQ(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1), "rpt must be 1..4 (1 for FP64)");
     QREQ((t.xcd == 0 || t.xcd == 1) && (t.gm == 0 || t.gm == 1), "xcd and group order are 0 or 1");

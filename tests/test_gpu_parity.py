@@ -285,7 +285,7 @@ def test_tuned_shape_keeps_bits(q, prec):
 
 
 @pytest.mark.parametrize("shape", [None, (256, 4, 3, 1, 0, 0, 0), (128, 2, 2, 1, 0, 0, 1),
-                                   (192, 3, 1, 1, 0, 0, 0, 0, 4, 2), (64, 1, 1, 0, 0, 0, 1, 0, 2, 4)])
+                                   (192, 3, 1, 1, 0, 0, 0, 0, 4, 2, 1), (64, 1, 1, 0, 0, 0, 1, 0, 2, 4, 0)])
 def test_batch12_clover_mixed(q, shape):
     """12 right-hand sides in one batched FP32-inner clover solve (default launch
     shape, or several right-hand sides per thread in either group order): every one
@@ -316,11 +316,11 @@ def test_hop_shapes_keep_bits(q, prec):
     g = q.Gauge.random(L, seed=34, eps=0.3)
     vs = [orc.source_gauss(L, 80 + r)[1] for r in range(12)]
     singles = [q.hop(g, v, 0, dagger=0, t_boundary=-1, recon=12, precision=prec) for v in vs]
-    # (block, rb, rpt, xcd, tile_z, tile_t, group_major, tile_y, rpw, spt)
-    shapes = [(256, 4, 1, 1, 4, 4, 1, 0, 1, 1), (128, 2, 3, 1, 4, 4, 1, 8, 1, 2), (64, 1, 2, 0, 0, 0, 1, 0, 1, 1),
-              (256, 2, 1, 1, 8, 8, 0, 8, 1, 4), (128, 1, 4, 0, 4, 4, 0, 0, 1, 1), (256, 4, 1, 1, 2, 4, 0, 8, 1, 3),
-              (192, 3, 1, 1, 4, 4, 0, 0, 4, 1), (64, 1, 1, 1, 0, 0, 1, 0, 4, 2), (128, 2, 1, 0, 4, 4, 1, 8, 2, 4),
-              (256, 4, 1, 1, 0, 0, 0, 0, 4, 1), (256, 2, 1, 1, 8, 8, 1, 0, 2, 2)]
+    # (block, rb, rpt, xcd, tile_z, tile_t, group_major, tile_y, rpw, spt, cached)
+    shapes = [(256, 4, 1, 1, 4, 4, 1, 0, 1, 1, 0), (128, 2, 3, 1, 4, 4, 1, 8, 1, 2, 1), (64, 1, 2, 0, 0, 0, 1, 0, 1, 1, 0),
+              (256, 2, 1, 1, 8, 8, 0, 8, 1, 4, 1), (128, 1, 4, 0, 4, 4, 0, 0, 1, 1, 0), (256, 4, 1, 1, 2, 4, 0, 8, 1, 3, 0),
+              (192, 3, 1, 1, 4, 4, 0, 0, 4, 1, 1), (64, 1, 1, 1, 0, 0, 1, 0, 4, 2, 0), (128, 2, 1, 0, 4, 4, 1, 8, 2, 4, 1),
+              (256, 4, 1, 1, 0, 0, 0, 0, 4, 1, 0), (256, 2, 1, 1, 8, 8, 1, 0, 2, 2, 1)]
     for sh in shapes:
         shape = dict(zip(q.SHAPE_KEYS, sh))
         if prec == 0:
