@@ -122,10 +122,13 @@ def serving_section(q, n_requests=240, seed=2024):
     def submit(server, r):
         return server.submit(gauges[r["g"]], r["kappa"], r["csw"], r["tol"], r["nrhs"], seed=r["seed"])
 
-    # warm-up: build packed gauges, tune the common shapes
-    for gi in range(8):
+    # warm-up: build packed gauges, tune the common shapes (tuned shapes are kept per
+    # volume, so one gauge of each volume is enough)
+    for gi in range(0, 8, 2):
+        t1 = time.perf_counter()
         q.tune_hop(gauges[gi], 1, 12, 12)
         q.tune_hop(gauges[gi], 1, 12, 1)
+        log(f"serving warm-up: tuned {'x'.join(map(str, vols[gi // 2]))} in {time.perf_counter() - t1:.1f} s")
     server = q.Server(precision=1, max_batch=12, chunk=8)
     for r in draw(8):
         submit(server, r)
@@ -145,11 +148,15 @@ def serving_section(q, n_requests=240, seed=2024):
     arrivals = np.cumsum(gaps)
     ids = []
     t0 = time.perf_counter()
-    for r, ta in zip(reqs, arrivals):
+    log(f"serving: capacity {cap:.1f} requests/s from a burst of {len(burst)}; "
+        f"{n_requests} Poisson arrivals at {rate:.1f}/s")
+    for k, (r, ta) in enumerate(zip(reqs, arrivals)):
         d = ta - (time.perf_counter() - t0)
         if d > 0:
             time.sleep(d)
         ids.append(submit(server, r))
+        if (k + 1) % 60 == 0:
+            log(f"serving: {k + 1} submitted at {time.perf_counter() - t0:.1f} s")
     server.drain()
     tend = time.perf_counter() - t0
     sts = [server.poll(i) for i in ids]
@@ -175,6 +182,7 @@ def serving_section(q, n_requests=240, seed=2024):
     # class (gauge, Wilson or clover, tol) at kappa = 0.117, then the whole stream is
     # replayed through a FIFO queue with the same arrival times, each request taking
     # nrhs times its class's solve time
+    log(f"serving: stream drained at {tend:.1f} s; timing the plain FP64 baseline per class")
     sp = q.default_solve_params(tol=1e-10, precision=0, graph=0)
     cls_time = {}
     for gi in range(8):

@@ -100,6 +100,18 @@ int qcuSourceGaussian(void* v, const int lattice_size[4], uint64_t seed);
 /* unit point source: 1 at full coordinates `coords` (x,y,z,t), spin, colour; 0 elsewhere */
 int qcuSourcePoint(void* v, const int lattice_size[4], const int coords[4], int spin, int colour);
 
+/* ---- layout converters ------------------------------------------------------ */
+/* Lexicographic <-> even-odd site order, complex128, for a caller whose fields are
+ * lexicographic (the reference takes even-odd only; its outside caller converts):
+ *   QCU_FIELD_SPINOR  [V][4][3]     <-> full spinor [2][Vh][4][3]
+ *   QCU_FIELD_GAUGE   [4][V][3][3]  <-> gauge [4][2][Vh][3][3]
+ * Lexicographic site l = ((t*Lz+z)*Ly+y)*Lx+x goes to parity (x+y+z+t)%2, half-site
+ * index l/2 (include/qcu_point.cuh:20-93).  dst and src must not overlap.  The
+ * reference's AoS <-> coalesced SoA shifts are in qcu_shift_storage.h. */
+enum { QCU_FIELD_SPINOR = 0, QCU_FIELD_GAUGE = 1 };
+int qcuLexToEvenOdd(void* eo, const void* lex, const int lattice_size[4], int field);
+int qcuEvenOddToLex(void* lex, const void* eo, const int lattice_size[4], int field);
+
 /* ---- operators (interface layouts) --------------------------------------- */
 /* out(parity) = D^s in(1-parity), s = -1 if dagger; precision selects the kernel
  * (inputs are rounded to it); recon 12 or 18 */

@@ -60,6 +60,9 @@ def _declare(L):
     L.orc_neighbor.argtypes = [_ip, ctypes.c_int, ctypes.c_long, ctypes.c_int, ctypes.c_int, _ip]
     L.orc_neighbor.restype = ctypes.c_long
     L.orc_reconstruct12.argtypes = [_dp]
+    L.orc_shift_vector.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_long]
+    L.orc_shift_gauge.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_long]
+    L.orc_lex_eo.argtypes = [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.orc_hop.argtypes = [_dp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     L.orc_clover_T.argtypes = [_dp, _dp, _ip, ctypes.c_int]
     L.orc_clover_A.argtypes = [_dp, _dp, ctypes.c_long, ctypes.c_double]
@@ -177,6 +180,40 @@ def reconstruct12(u):
     u = np.ascontiguousarray(u, np.complex128).copy()
     lib().orc_reconstruct12(_p(u))
     return u
+
+
+# ---- layouts -----------------------------------------------------------------
+def shift_vector(src, direction, L):
+    """Half spinor: complex128 [Vh][4][3] -> float64 [24][Vh] (direction 0) or back."""
+    vh = volume(L) // 2
+    src = np.ascontiguousarray(src)
+    if direction == 0:
+        out = np.empty((24, vh), np.float64)
+    else:
+        out = np.empty((vh, 4, 3), np.complex128)
+    lib().orc_shift_vector(_p(out), _p(src), direction, vh)
+    return out
+
+
+def shift_gauge(src, direction, L):
+    """Gauge: complex128 [4][2][Vh][3][3] -> float64 [4][2][12][Vh] (direction 0) or
+    back with row 2 rebuilt."""
+    vh = volume(L) // 2
+    src = np.ascontiguousarray(src)
+    if direction == 0:
+        out = np.empty((4, 2, 12, vh), np.float64)
+    else:
+        out = np.empty((4, 2, vh, 3, 3), np.complex128)
+    lib().orc_shift_gauge(_p(out), _p(src), direction, vh)
+    return out
+
+
+def lex_eo(src, L, nc, nf, to_eo):
+    """nf fields of nc complex per site, lexicographic <-> even-odd (same shape out)."""
+    src = np.ascontiguousarray(src, np.complex128)
+    out = np.empty_like(src)
+    lib().orc_lex_eo(_p(out), _p(src), _L(L), nc, nf, int(to_eo))
+    return out
 
 
 # ---- operators ---------------------------------------------------------------

@@ -133,6 +133,59 @@ void orc_reconstruct12(double* u) { recon12((cpx*)u); }
 
 static inline long vh_of(const int L[4]) { return (long)L[0] * L[1] * L[2] * L[3] / 2; }
 
+/* ------------------------------------------------------------------------- */
+/* layouts                                                                    */
+/* ------------------------------------------------------------------------- */
+/* src/qcu_shift_storage.cu:74-116,166-180: one half spinor, AoS [Vh][24] doubles
+ * <-> coalesced [24][Vh]; dir 0 = to coalesced */
+void orc_shift_vector(double* dst, const double* src, int dir, long vh) {
+  for (long s = 0; s < vh; ++s)
+    for (int i = 0; i < 24; ++i) {
+      if (dir == 0) dst[i * vh + s] = src[s * 24 + i];
+      else dst[s * 24 + i] = src[i * vh + s];
+    }
+}
+
+/* src/qcu_shift_storage.cu:123-161,183-194: per (mu, parity) field, links [Vh][18]
+ * doubles -> rows 0-1 as [12][Vh] (dir 0).  dir 1 (left empty by the reference) goes
+ * back and rebuilds row 2 as recon12 does. */
+void orc_shift_gauge(double* dst, const double* src, int dir, long vh) {
+  for (int f = 0; f < 8; ++f)
+    for (long s = 0; s < vh; ++s) {
+      if (dir == 0) {
+        const double* a = src + ((long)f * vh + s) * 18;
+        for (int j = 0; j < 12; ++j) dst[((long)f * 12 + j) * vh + s] = a[j];
+      } else {
+        cpx u[9];
+        double* d = (double*)u;
+        for (int j = 0; j < 12; ++j) d[j] = src[((long)f * 12 + j) * vh + s];
+        recon12(u);
+        memcpy(dst + ((long)f * vh + s) * 18, u, sizeof u);
+      }
+    }
+}
+
+/* lexicographic site ((t*Lz+z)*Ly+y)*Lx+x <-> even-odd (parity, half-site index) of
+ * orc_coords; nf fields of nc complex per site.  to_eo: lex -> even-odd. */
+void orc_lex_eo(double* dst, const double* src, const int L[4], int nc, int nf, int to_eo) {
+  long vh = vh_of(L);
+  const cpx* a = (const cpx*)src;
+  cpx* b = (cpx*)dst;
+  for (int f = 0; f < nf; ++f)
+    for (int p = 0; p < 2; ++p)
+      for (long i = 0; i < vh; ++i) {
+        int c[4];
+        orc_coords(L, p, i, c);
+        long l = (((long)c[3] * L[2] + c[2]) * L[1] + c[1]) * L[0] + c[0];
+        long e = (long)f * 2 * vh + p * vh + i;
+        long x = (long)f * 2 * vh + l;
+        for (int k = 0; k < nc; ++k) {
+          if (to_eo) b[e * nc + k] = a[x * nc + k];
+          else b[x * nc + k] = a[e * nc + k];
+        }
+      }
+}
+
 /* link U_mu at (parity, idx); include/qcu_point.cuh:65-67 */
 static void load_link(cpx* u, const double* gauge, const int L[4], int mu, int parity, long idx, int recon) {
   const cpx* g = (const cpx*)gauge + ((((long)mu * 2 + parity) * vh_of(L)) + idx) * 9;

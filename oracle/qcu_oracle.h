@@ -24,6 +24,8 @@
  *   orc_full_op/orc_schur_* dense (A - kappa D) from the textbook hop; Schur identity
  *   orc_cg_eo / orc_cg_ref  numpy.linalg.solve of the dense operator on 4^4
  *   orc_gauge_random etc.   unitarity / det = 1, seeded determinism, moments
+ *   orc_shift_*, orc_lex_eo coordinate-coded known answers (numpy lexicographic grid),
+ *                           round trips, SU(3) row-2 closed form
  */
 #ifndef QCU_ORACLE_H
 #define QCU_ORACLE_H
@@ -44,6 +46,17 @@ long orc_index(const int L[4], const int c[4], int* parity);
  * half-site index; its parity is 1-parity.  *wrapped = 1 if the step crossed the
  * lattice edge in direction mu. */
 long orc_neighbor(const int L[4], int parity, long idx, int mu, int fwd, int* wrapped);
+
+/* ---- layouts (complex128 fields as doubles) ------------------------------ */
+/* one half spinor [Vh][24] <-> the reference's coalesced [24][Vh]; dir 0 = to it
+ * (src/qcu_shift_storage.cu:74-116) */
+void orc_shift_vector(double* dst, const double* src, int dir, long vh);
+/* gauge [8][Vh][18] -> coalesced rows 0-1 [8][12][Vh] (dir 0,
+ * src/qcu_shift_storage.cu:123-161); dir 1 back, rebuilding row 2 */
+void orc_shift_gauge(double* dst, const double* src, int dir, long vh);
+/* nf fields of nc complex per site: lexicographic [nf][V][nc] <-> even-odd
+ * [nf][2][Vh][nc] (to_eo selects) */
+void orc_lex_eo(double* dst, const double* src, const int L[4], int nc, int nf, int to_eo);
 
 /* ---- SU(3) ---------------------------------------------------------------- */
 /* row2 = conj(row0 x row1), src/qcu_wilson_dslash_neo.cu:73-78 */

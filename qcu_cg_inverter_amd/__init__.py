@@ -93,7 +93,7 @@ _dbl = ctypes.c_double
 _sz = ctypes.c_size_t
 _i4 = ctypes.POINTER(ctypes.c_int)
 
-# name -> (restype, argtypes); every symbol declared in include/qcu.h and include/qcu_ex.h
+# name -> (restype, argtypes); every symbol declared in include/*.h
 PROTOTYPES = {
     "initGridSize": (None, [ctypes.POINTER(QcuGrid), ctypes.POINTER(QcuParam), _vp, _vp, _vp]),
     "dslashQcu": (None, [_vp, _vp, _vp, ctypes.POINTER(QcuParam), _int]),
@@ -121,6 +121,10 @@ PROTOTYPES = {
     "qcuCloverExport": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _i4]),
     "qcuSourceGaussian": (_int, [_vp, _i4, ctypes.c_uint64]),
     "qcuSourcePoint": (_int, [_vp, _i4, _i4, _int, _int]),
+    "qcuLexToEvenOdd": (_int, [_vp, _vp, _i4, _int]),
+    "qcuEvenOddToLex": (_int, [_vp, _vp, _i4, _int]),
+    "shiftVectorStorage": (None, [_vp, _vp, _int, _int, _int, _int, _int]),
+    "shiftGaugeStorage": (None, [_vp, _vp, _int, _int, _int, _int, _int]),
     "qcuHop": (_int, [_vp, _vp, _vp, _int, _int, _int, _int, _int]),
     "qcuHopBatch": (_int, [_vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _int, _int, _int, _int, _int]),
     "qcuApplyM": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int]),
@@ -374,6 +378,57 @@ def source_point_device(L, coords, spin, colour) -> DeviceBuffer:
     buf = DeviceBuffer(2 * half_volume(L) * 192)
     check(lib().qcuSourcePoint(buf.ptr, _L(L), _L(coords), spin, colour))
     return buf
+
+
+FIELD_SPINOR, FIELD_GAUGE = 0, 1
+TO_COALESCE, TO_NON_COALESCE = 0, 1
+
+
+def lex_to_eo(lex: np.ndarray, L, field=FIELD_SPINOR) -> np.ndarray:
+    """Lexicographic -> even-odd order on the device (qcuLexToEvenOdd); complex128."""
+    a = DeviceBuffer.from_numpy(np.ascontiguousarray(lex, np.complex128))
+    o = DeviceBuffer(a.nbytes)
+    check(lib().qcuLexToEvenOdd(o.ptr, a.ptr, _L(L), field))
+    return o.to_numpy(lex.shape)
+
+
+def eo_to_lex(eo: np.ndarray, L, field=FIELD_SPINOR) -> np.ndarray:
+    a = DeviceBuffer.from_numpy(np.ascontiguousarray(eo, np.complex128))
+    o = DeviceBuffer(a.nbytes)
+    check(lib().qcuEvenOddToLex(o.ptr, a.ptr, _L(L), field))
+    return o.to_numpy(eo.shape)
+
+
+def shift_vector_storage(src: np.ndarray, direction, L) -> np.ndarray:
+    """The reference's shiftVectorStorage on one half spinor: complex128 [Vh][4][3]
+    <-> float64 [24][Vh]."""
+    vh = half_volume(L)
+    a = DeviceBuffer.from_numpy(np.ascontiguousarray(src))
+    o = DeviceBuffer(vh * 192)
+    lib().qcuClearLastError()
+    lib().shiftVectorStorage(o.ptr, a.ptr, direction, *L)
+    check(lib().qcuGetLastError())
+    if direction == TO_COALESCE:
+        return o.to_numpy((24, vh), np.float64)
+    return o.to_numpy((vh, 4, 3))
+
+
+def shift_gauge_storage(src: np.ndarray, direction, L) -> np.ndarray:
+    """The reference's shiftGaugeStorage: complex128 [4][2][Vh][3][3] <-> float64
+    [4][2][12][Vh] (rows 0-1; the way back rebuilds row 2)."""
+    vh = half_volume(L)
+    a = DeviceBuffer.from_numpy(np.ascontiguousarray(src))
+    if direction == TO_COALESCE:
+        o = DeviceBuffer(8 * vh * 12 * 8)
+        lib().qcuClearLastError()
+        lib().shiftGaugeStorage(o.ptr, a.ptr, direction, *L)
+        check(lib().qcuGetLastError())
+        return o.to_numpy((4, 2, 12, vh), np.float64)
+    o = DeviceBuffer(8 * vh * 18 * 8)
+    lib().qcuClearLastError()
+    lib().shiftGaugeStorage(o.ptr, a.ptr, direction, *L)
+    check(lib().qcuGetLastError())
+    return o.to_numpy((4, 2, vh, 3, 3))
 
 
 def source_gaussian_device(L, seed) -> DeviceBuffer:

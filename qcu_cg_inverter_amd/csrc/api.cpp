@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "qcu_host.h"
+#include "qcu_shift_storage.h"
 
 namespace qcu {
 
@@ -376,8 +377,64 @@ extern "C" void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuPara
   }
 }
 
+// ==================================================== qcu_shift_storage.h ===
+static void require_disjoint(const void* a, const void* b, size_t n, const char* who) {
+  const char* x = static_cast<const char*>(a);
+  const char* y = static_cast<const char*>(b);
+  QREQ(x + n <= y || y + n <= x, std::string(who) + ": dst and src overlap");
+}
+
+// kind 0: one half spinor, kind 1: a gauge (src/qcu_shift_storage.cu:166-194)
+static void shift_storage(const char* who, int kind, void* dst, void* src, int dir, int Lx, int Ly, int Lz, int Lt) {
+  ensure_init();
+  QREQ(dst && src, std::string(who) + ": NULL pointer");
+  QREQ(dir == TO_COALESCE || dir == TO_NON_COALESCE, std::string(who) + ": shift_direction must be 0 or 1");
+  const int L[4] = {Lx, Ly, Lz, Lt};
+  validate_lattice(L);
+  long vh = (long)Lx * Ly * Lz * Lt / 2;
+  size_t aos = (size_t)vh * (kind == 0 ? 24 : 8 * 18) * 8;
+  size_t soa = (size_t)vh * (kind == 0 ? 24 : 8 * 12) * 8;
+  require_disjoint(dst, src, aos > soa ? aos : soa, who);
+  launch_shift_storage(kind, dir == TO_COALESCE, src, dst, vh, stream());
+  QHIP(hipGetLastError());
+  QHIP(hipStreamSynchronize(stream()));
+}
+
+extern "C" void shiftVectorStorage(void* dst_vec, void* src_vec, int shift_direction, int Lx, int Ly, int Lz, int Lt) {
+  dropin("shiftVectorStorage",
+         [&] { shift_storage("shiftVectorStorage", 0, dst_vec, src_vec, shift_direction, Lx, Ly, Lz, Lt); });
+}
+
+extern "C" void shiftGaugeStorage(void* dst_vec, void* src_vec, int shift_direction, int Lx, int Ly, int Lz, int Lt) {
+  dropin("shiftGaugeStorage",
+         [&] { shift_storage("shiftGaugeStorage", 1, dst_vec, src_vec, shift_direction, Lx, Ly, Lz, Lt); });
+}
+
 // ================================================================ qcu_ex.h ===
 extern "C" {
+
+static int lex_eo(const char* who, int to_eo, void* dst, const void* src, const int L[4], int field) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(dst && src && L, std::string(who) + ": NULL argument");
+    QREQ(field == QCU_FIELD_SPINOR || field == QCU_FIELD_GAUGE, std::string(who) + ": field must be spinor or gauge");
+    validate_lattice(L);
+    long vol = (long)L[0] * L[1] * L[2] * L[3];
+    QREQ(vol < 2147483648L, std::string(who) + ": volume must be below 2^31 sites");
+    int nc = field == QCU_FIELD_SPINOR ? 12 : 9, nf = field == QCU_FIELD_SPINOR ? 1 : 4;
+    require_disjoint(dst, src, (size_t)vol * nc * nf * 16, who);
+    launch_lex_eo(to_eo, src, dst, L, nc, nf, stream());
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qcuLexToEvenOdd(void* eo, const void* lex, const int L[4], int field) {
+  return lex_eo("qcuLexToEvenOdd", 1, eo, lex, L, field);
+}
+int qcuEvenOddToLex(void* lex, const void* eo, const int L[4], int field) {
+  return lex_eo("qcuEvenOddToLex", 0, lex, eo, L, field);
+}
 
 int qcuInit(int device) {
   return guarded([&] {

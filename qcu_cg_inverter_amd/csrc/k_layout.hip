@@ -126,6 +126,118 @@ void launch_gauge_reconstruct(void* aos, long vh, hipStream_t st) {
   hipLaunchKernelGGL(gauge_reconstruct_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, static_cast<double2*>(aos), n);
 }
 
+// ----------------------------------------------- reference storage shifts ---
+// AoS <-> the reference's coalesced SoA (src/qcu_shift_storage.cu:74-161): an AoS
+// record of `wa` doubles per site becomes `w` planes of n doubles (plane e holds
+// element e of every site).  A block moves 64 sites through an LDS tile, so both the
+// record reads and the plane writes are contiguous runs; blockIdx.y picks the field
+// (the 8 (mu, parity) link fields of a gauge, or 1 for a half spinor).
+constexpr int kTile = 64, kTilePad = kTile + 1, kTileMaxW = 24;
+
+__global__ void __launch_bounds__(256) aos_to_soa_kernel(const double* src, double* dst, long n, int wa, int w,
+                                                         long src_field, long dst_field) {
+  __shared__ double tile[kTileMaxW * kTilePad];
+  long s0 = (long)blockIdx.x * kTile;
+  int ns = (int)min((long)kTile, n - s0);
+  src += blockIdx.y * src_field + s0 * wa;
+  dst += blockIdx.y * dst_field + s0;
+  for (int j = threadIdx.x; j < ns * wa; j += blockDim.x) {
+    int e = j % wa, s = j / wa;
+    if (e < w) tile[e * kTilePad + s] = src[j];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kTile * w; j += blockDim.x) {
+    int e = j / kTile, s = j % kTile;
+    if (s < ns) dst[e * n + s] = tile[e * kTilePad + s];
+  }
+}
+
+// the way back; RECON rebuilds link row 2 = conj(row0 x row1) from the 12 stored reals
+template <bool RECON>
+__global__ void __launch_bounds__(256) soa_to_aos_kernel(const double* src, double* dst, long n, int w, int wa,
+                                                         long src_field, long dst_field) {
+  __shared__ double tile[kTileMaxW * kTilePad];
+  long s0 = (long)blockIdx.x * kTile;
+  int ns = (int)min((long)kTile, n - s0);
+  src += blockIdx.y * src_field + s0;
+  dst += blockIdx.y * dst_field + s0 * wa;
+  for (int j = threadIdx.x; j < kTile * w; j += blockDim.x) {
+    int e = j / kTile, s = j % kTile;
+    if (s < ns) tile[e * kTilePad + s] = src[e * n + s];
+  }
+  __syncthreads();
+  if (RECON) {
+    if ((int)threadIdx.x < ns) {
+      int s = threadIdx.x;
+      cx<double> u[9];
+#pragma unroll
+      for (int e = 0; e < 6; ++e) u[e] = {tile[(2 * e) * kTilePad + s], tile[(2 * e + 1) * kTilePad + s]};
+      recon_row2(u);
+#pragma unroll
+      for (int e = 6; e < 9; ++e) {
+        tile[(2 * e) * kTilePad + s] = u[e].re;
+        tile[(2 * e + 1) * kTilePad + s] = u[e].im;
+      }
+    }
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < ns * wa; j += blockDim.x) dst[j] = tile[(j % wa) * kTilePad + j / wa];
+}
+
+void launch_shift_storage(int kind, int to_soa, const void* src, void* dst, long vh, hipStream_t st) {
+  // kind 0: half spinor [Vh][24] <-> [24][Vh]; kind 1: gauge [8][Vh][18] <-> [8][12][Vh]
+  int wa = kind == 0 ? 24 : 18, w = kind == 0 ? 24 : 12, nf = kind == 0 ? 1 : 8;
+  dim3 grid(nblk(vh, kTile), nf);
+  const double* s = static_cast<const double*>(src);
+  double* d = static_cast<double*>(dst);
+  if (to_soa)
+    hipLaunchKernelGGL(aos_to_soa_kernel, grid, dim3(256), 0, st, s, d, vh, wa, w, vh * wa, vh * w);
+  else if (kind == 0)
+    hipLaunchKernelGGL(soa_to_aos_kernel<false>, grid, dim3(256), 0, st, s, d, vh, w, wa, vh * w, vh * wa);
+  else
+    hipLaunchKernelGGL(soa_to_aos_kernel<true>, grid, dim3(256), 0, st, s, d, vh, w, wa, vh * w, vh * wa);
+}
+
+// ------------------------------------------- lexicographic <-> even-odd ---
+// lexicographic site l = ((t*Lz + z)*Ly + y)*Lx + x  <->  even-odd (parity, l/2) with
+// parity = (x+y+z+t) mod 2, the half-site order of include/qcu_point.cuh:20-93.  One
+// thread per 16-byte complex of a site record (NC complex per site), so consecutive
+// lanes move consecutive bytes of the lexicographic field; blockIdx.y picks the field.
+// The launcher guarantees vol < 2^31, the range of FastDiv.
+template <bool TO_EO, int NC>
+__global__ void lex_eo_kernel(const double2* src, double2* dst, long vol, FastDiv dlx, FastDiv dly, FastDiv dlz,
+                              long field) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= vol * NC) return;
+  uint32_t l = (uint32_t)(j / NC);
+  int k = (int)(j - (long)l * NC);
+  uint32_t r = dlx.div(l), x = l - r * dlx.d;
+  uint32_t r2 = dly.div(r), y = r - r2 * dly.d;
+  uint32_t t = dlz.div(r2), z = r2 - t * dlz.d;
+  uint32_t parity = (x + y + z + t) & 1u;
+  long e = (long)parity * (vol / 2) + (l >> 1);
+  src += blockIdx.y * field;
+  dst += blockIdx.y * field;
+  if (TO_EO) dst[e * NC + k] = src[(long)l * NC + k];
+  else dst[(long)l * NC + k] = src[e * NC + k];
+}
+
+void launch_lex_eo(int to_eo, const void* src, void* dst, const int L[4], int nc, int nfields, hipStream_t st) {
+  long vol = (long)L[0] * L[1] * L[2] * L[3];
+  long n = vol * nc;
+  dim3 grid(nblk(n, 256), nfields);
+  const double2* s = static_cast<const double2*>(src);
+  double2* d = static_cast<double2*>(dst);
+  FastDiv dlx((uint32_t)L[0]), dly((uint32_t)L[1]), dlz((uint32_t)L[2]);
+#define QCU_LEX(E, N)                                                                                   \
+  if (to_eo == E && nc == N) {                                                                          \
+    hipLaunchKernelGGL((lex_eo_kernel<E, N>), grid, dim3(256), 0, st, s, d, vol, dlx, dly, dlz, n);     \
+    return;                                                                                             \
+  }
+  QCU_LEX(1, 12) QCU_LEX(0, 12) QCU_LEX(1, 9) QCU_LEX(0, 9)
+#undef QCU_LEX
+}
+
 template <int PI, int PO>
 __global__ void spinor_convert_kernel(Vec in, Vec out, long vh, int accumulate) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

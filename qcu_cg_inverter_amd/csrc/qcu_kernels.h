@@ -29,6 +29,11 @@ void launch_spinor_to_aos(int prec, Vec in, void* aos, long aos_stride, long vh,
 void launch_gauge_pack(int prec, int recon, const void* aos, void* out, long vh, hipStream_t st);
 // rebuild row 2 of every interface-layout link from rows 0 and 1 (in place)
 void launch_gauge_reconstruct(void* aos, long vh, hipStream_t st);
+// reference storage shifts: kind 0 half spinor [Vh][24] <-> [24][Vh] doubles; kind 1
+// gauge [8][Vh][18] -> [8][12][Vh] (back with row 2 rebuilt)
+void launch_shift_storage(int kind, int to_soa, const void* src, void* dst, long vh, hipStream_t st);
+// lexicographic <-> even-odd, nc (12 or 9) complex per site, nfields fields of V sites
+void launch_lex_eo(int to_eo, const void* src, void* dst, const int L[4], int nc, int nfields, hipStream_t st);
 // out = in (accumulate = 0) or out += in (accumulate = 1), any precisions
 void launch_spinor_convert(int pin, Vec in, int pout, Vec out, long vh, int nrhs, int accumulate, hipStream_t st);
 
