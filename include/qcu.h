@@ -54,7 +54,10 @@ void fullDslashQcu(void* fermion_out, void* fermion_in, void* gauge, QcuParam* p
 /* Replaces cg_inverter, include/qcu.h:18 (src/cg_inverter.cu:260-369).
  * Full spinors: x_vector <- (A - D)^-1 b_vector; b_vector is not modified.  Solved
  * by even-odd preconditioned CG on the normal equations to a true relative residual
- * ||b - M x|| / ||b|| of 1e-12 (qcuSetDefaultSolve in qcu_ex.h changes it). */
+ * ||b - M x|| / ||b|| of 1e-12 within 20 Vh iterations (qcuSetDefaultSolve in
+ * qcu_ex.h changes both).  As in the reference, a solve that does not converge is
+ * not fatal: x holds the last iterate, "cg_failed!!!" is printed and
+ * qcuGetLastError() returns QCU_ERROR_NOT_CONVERGED. */
 void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuParam* param);
 
 /* Replaces loadQcuGauge, include/qcu.h:20 (src/dslash.cu:27-35).

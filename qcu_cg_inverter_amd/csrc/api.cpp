@@ -359,7 +359,11 @@ extern "C" void fullDslashQcu(void* fermion_out, void* fermion_in, void* gauge, 
 extern "C" void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuParam* param) {
   qcuSolveStats st;
   memset(&st, 0, sizeof st);
-  dropin("cg_inverter", [&] {
+  // A solve that stops at max_iter (or breaks down) is not an error of the call: like
+  // the reference (src/cg_inverter.cu:331-345) it leaves the last iterate in x and
+  // prints cg_failed!!!; qcuGetLastError() then returns QCU_ERROR_NOT_CONVERGED (or
+  // QCU_ERROR_BREAKDOWN).  The solve has synchronised by then.
+  int rc = guarded([&] {
     ensure_init();
     QREQ(b_vector && x_vector, "cg_inverter: NULL spinor");
     GaugeField& g = dropin_gauge(gauge, param, false);
@@ -369,6 +373,10 @@ extern "C" void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuPara
     solve_batch(g, op, ctx().default_solve, 1, xs, bs, &st);
     QHIP(hipStreamSynchronize(stream()));
   });
+  if (rc != QCU_SUCCESS && rc != QCU_ERROR_NOT_CONVERGED && rc != QCU_ERROR_BREAKDOWN) {
+    const std::string msg = g_last_msg;
+    dropin("cg_inverter", [&] { throw Error(rc, msg); });
+  }
   if (!quiet()) {
     // the reference's progress lines (src/cg_inverter.cu:330,342-349)
     if (st.converged) printf("number of iteration %d\ncg success!!!\n", st.iterations);

@@ -44,6 +44,7 @@ def test_dropin_symbols_on_golden(q, gold):
     vh = L[0] * L[1] * L[2] * L[3] // 2
     prm = q.param(L)
     gd, b = dev(q, gold["gauge"]), gold["b"]
+    q.lib().qcuClearLastError()
     q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
     q.lib().loadQcuGauge(gd.ptr, ctypes.byref(prm))
     for parity, key in ((0, "dslash_p0"), (1, "dslash_p1")):
@@ -55,12 +56,43 @@ def test_dropin_symbols_on_golden(q, gold):
         out = q.DeviceBuffer(2 * vh * 192)
         q.lib().fullDslashQcu(out.ptr, inp.ptr, gd.ptr, ctypes.byref(prm), dag)
         assert rel(out.to_numpy((2, vh, 4, 3)), gold[key]) < FP64_TOL, key
+    assert q.lib().qcuGetLastError() == 0, q.lib().qcuGetLastErrorString()
+    # kappa = 1 is far past the critical kappa, so this system is ill-conditioned: it
+    # takes about 1e4 CG iterations, more than cg_inverter's default cap of 20 Vh
     xd = q.DeviceBuffer(2 * vh * 192)
-    q.lib().cg_inverter(inp.ptr, xd.ptr, gd.ptr, ctypes.byref(prm))
-    assert q.lib().qcuGetLastError() == 0
-    # kappa = 1 is far past the critical kappa, so this system is ill-conditioned
-    # (about 1e4 CG iterations): the solution agrees to what 1e-12 residuals allow
+    sp = q.default_solve_params(tol=1e-12, precision=q.PREC_DOUBLE, max_iter=40000)
+    q.check(q.lib().qcuSetDefaultSolve(ctypes.byref(sp)))
+    try:
+        q.lib().cg_inverter(inp.ptr, xd.ptr, gd.ptr, ctypes.byref(prm))
+        assert q.lib().qcuGetLastError() == 0, q.lib().qcuGetLastErrorString()
+    finally:
+        sp.max_iter = 0
+        q.check(q.lib().qcuSetDefaultSolve(ctypes.byref(sp)))
+    # the solution agrees to what 1e-12 residuals allow
     assert rel(xd.to_numpy((2, vh, 4, 3)), gold["x_ref"]) < 1e-8
+
+
+def test_dropin_cg_inverter_cap_reports_failure(q, gold):
+    """Stopping at the iteration cap leaves the last iterate and reports, as the
+    reference's cg_failed!!! path does, without ending the process."""
+    L = tuple(gold["meta"]["L"])
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    prm = q.param(L)
+    gd, inp = dev(q, gold["gauge"]), dev(q, gold["b"])
+    xd = q.DeviceBuffer(2 * vh * 192)
+    sp = q.default_solve_params(tol=1e-12, precision=q.PREC_DOUBLE, max_iter=50)
+    q.check(q.lib().qcuSetDefaultSolve(ctypes.byref(sp)))
+    q.lib().qcuClearLastError()
+    try:
+        q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+        q.lib().cg_inverter(inp.ptr, xd.ptr, gd.ptr, ctypes.byref(prm))
+        assert q.lib().qcuGetLastError() == 4  # QCU_ERROR_NOT_CONVERGED
+    finally:
+        sp.max_iter = 0
+        q.check(q.lib().qcuSetDefaultSolve(ctypes.byref(sp)))
+        q.lib().qcuClearLastError()
+    x = xd.to_numpy((2, vh, 4, 3))
+    assert np.isfinite(x).all() and np.linalg.norm(x) > 0
 
 
 def test_clover_export_on_golden(q, gold):
