@@ -50,7 +50,8 @@ typedef struct {
   double tol;      /* stop when ||b - M x|| / ||b|| <= tol (FP64, full lattice) */
   int max_iter;    /* iteration cap (<= 0: 10 * volume) */
   int precision;   /* inner precision: QCU_PREC_DOUBLE = plain FP64 CG */
-  double delta;    /* reliable update when |r| < delta * max |r| since the last one */
+  double delta;    /* reliable update when |r| < delta * |r| at the last one; outside
+                      (0, 1), as by default: 1e-3 for an FP32 inner loop, 0.1 for FP16 */
   int graph;       /* 1: replay captured hipGraphs of `chunk` iterations */
   int chunk;       /* iterations between host convergence checks */
 } qcuSolveParams;

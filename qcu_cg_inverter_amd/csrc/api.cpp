@@ -386,10 +386,11 @@ extern "C" void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuPara
 }
 
 // ==================================================== qcu_shift_storage.h ===
-static void require_disjoint(const void* a, const void* b, size_t n, const char* who) {
+// [a, a + na) and [b, b + nb) must not overlap
+static void require_disjoint(const void* a, size_t na, const void* b, size_t nb, const char* who) {
   const char* x = static_cast<const char*>(a);
   const char* y = static_cast<const char*>(b);
-  QREQ(x + n <= y || y + n <= x, std::string(who) + ": dst and src overlap");
+  QREQ(x + na <= y || y + nb <= x, std::string(who) + ": dst and src overlap");
 }
 
 // kind 0: one half spinor, kind 1: a gauge (src/qcu_shift_storage.cu:166-194)
@@ -402,8 +403,9 @@ static void shift_storage(const char* who, int kind, void* dst, void* src, int d
   long vh = (long)Lx * Ly * Lz * Lt / 2;
   size_t aos = (size_t)vh * (kind == 0 ? 24 : 8 * 18) * 8;
   size_t soa = (size_t)vh * (kind == 0 ? 24 : 8 * 12) * 8;
-  require_disjoint(dst, src, aos > soa ? aos : soa, who);
-  launch_shift_storage(kind, dir == TO_COALESCE, src, dst, vh, stream());
+  const bool to_soa = dir == TO_COALESCE;
+  require_disjoint(dst, to_soa ? soa : aos, src, to_soa ? aos : soa, who);
+  launch_shift_storage(kind, to_soa, src, dst, vh, stream());
   QHIP(hipGetLastError());
   QHIP(hipStreamSynchronize(stream()));
 }
@@ -430,7 +432,8 @@ static int lex_eo(const char* who, int to_eo, void* dst, const void* src, const 
     long vol = (long)L[0] * L[1] * L[2] * L[3];
     QREQ(vol < 2147483648L, std::string(who) + ": volume must be below 2^31 sites");
     int nc = field == QCU_FIELD_SPINOR ? 12 : 9, nf = field == QCU_FIELD_SPINOR ? 1 : 4;
-    require_disjoint(dst, src, (size_t)vol * nc * nf * 16, who);
+    const size_t n = (size_t)vol * nc * nf * 16;
+    require_disjoint(dst, n, src, n, who);
     launch_lex_eo(to_eo, src, dst, L, nc, nf, stream());
     QHIP(hipGetLastError());
     QHIP(hipStreamSynchronize(stream()));
@@ -689,7 +692,7 @@ void qcuDefaultSolveParams(qcuSolveParams* sp) {
   sp->tol = 1e-10;
   sp->max_iter = 0;
   sp->precision = QCU_PREC_SINGLE;
-  sp->delta = 0.1;
+  sp->delta = 0.0;  // by inner precision: 1e-3 for FP32, 0.1 for FP16
   sp->graph = 1;
   sp->chunk = 8;
 }

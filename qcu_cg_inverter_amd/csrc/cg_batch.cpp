@@ -24,7 +24,7 @@ CgBatch::CgBatch(GaugeField& g, int prec, int recon, int t_boundary, bool clover
   QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
   QREQ(t_boundary == 1 || t_boundary == -1, "t_boundary must be +1 or -1");
   if (sp_.chunk < 1) sp_.chunk = 1;
-  if (sp_.delta <= 0.0 || sp_.delta >= 1.0) sp_.delta = 0.1;
+  sp_.delta = reliable_delta(prec, sp_.delta);
   vh_ = g.geo.vh;
   nw_ = n_partials(vh_);
   fb_ = (long)spinor_bytes(F64, vh_);

@@ -148,10 +148,17 @@ struct SpinorIO<F32> {
       v[2 * k + 1] = {q.z, q.w};
     }
   }
+  // streaming (BLAS) loads of a site's own spinor: nontemporal
   static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
-    load_cached(base, vh, i, v);
+    const float4* b = static_cast<const float4*>(base) + i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      float4 q = ld_nt(b + k * vh);
+      v[2 * k] = {q.x, q.y};
+      v[2 * k + 1] = {q.z, q.w};
+    }
   }
-  template <int NT = 1>  // FP32 spinors are always loaded cached
+  template <int NT = 1>  // FP32 spinors are always loaded cached in the hop
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const float4* b = static_cast<const float4*>(base) + i + (long)(3 * h) * vh;
 #pragma unroll
@@ -204,11 +211,20 @@ struct SpinorIO<F16> {
       for (int j = 0; j < 4; ++j) v[4 * k + j] = {f[2 * j] * nrm, f[2 * j + 1] * nrm};
     }
   }
+  // streaming (BLAS) loads of a site's own spinor: nontemporal
   static __device__ __forceinline__ void load(const void* base, long vh, long i, cx<T> v[12]) {
-    load_cached(base, vh, i, v);
+    const uint4* b = static_cast<const uint4*>(base) + i;
+    float nrm = __uint_as_float(ld_nt(reinterpret_cast<const unsigned*>(norms(base, vh) + i)));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float f[8];
+      unpack8(ld_nt(b + k * vh), f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[4 * k + j] = {f[2 * j] * nrm, f[2 * j + 1] * nrm};
+    }
   }
   // components 0..5 are word 0 and the first half of word 1; 6..11 the rest
-  template <int NT = 1>  // FP16 spinors are always loaded cached
+  template <int NT = 1>  // FP16 spinors are always loaded cached in the hop
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const uint4* b = static_cast<const uint4*>(base) + i + (long)h * vh;
     float nrm = norms(base, vh)[i];

@@ -119,7 +119,7 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
   QREQ(sp.tol > 0.0 && std::isfinite(sp.tol), "tol must be positive");
   QREQ(sp.precision >= 0 && sp.precision <= 2, "bad precision");
   if (sp.chunk < 1) sp.chunk = 1;
-  if (sp.delta <= 0.0 || sp.delta >= 1.0) sp.delta = 0.1;
+  sp.delta = reliable_delta(sp.precision, sp.delta);
   for (int r = 0; r < nrhs; ++r) QREQ(x[r] && b[r], "NULL spinor");
   check_op(op);
   const int P = sp.precision;

@@ -100,7 +100,10 @@ def test_clover_export_on_golden(q, gold):
     L = tuple(m["L"])
     g = q.Gauge.from_numpy(gold["gauge"], L)
     A, Ainv, nf = g.clover(m["ref_kappa"], m["ref_csw"])
-    assert nf == 0
+    # at kappa * c_sw = 1 some 6x6 blocks are indefinite: Cholesky fails on them and
+    # the pivoted Gauss-Jordan fallback inverts them; 32 of the 512 blocks here, as
+    # the oracle counts
+    assert nf == 32
     assert rel(A, gold["clover_A_ref"]) < FP64_TOL
     assert rel(Ainv, gold["clover_Ainv_ref"]) < 1e-13
 
