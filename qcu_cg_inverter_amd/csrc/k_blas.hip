@@ -188,7 +188,33 @@ __global__ void cg_update_kernel(Vec x, Vec p, Vec t, Vec s, const CgScal* sc, d
   int k = blockIdx.y;
   if (!sc[k].active) return;
   double acc = 0.0;
-  if (i < vh) {
+  if (P == F64 && PX == F64 && i < vh) {
+    // FP64: the four spinors of a site in two halves of 6 components, which fits the
+    // registers (all 48 complex at once spill); same operations in the same order
+    const double alpha = sc[k].alpha;
+    const char* pp = static_cast<const char*>(p.p) + k * p.stride;
+    const char* tp = static_cast<const char*>(t.p) + k * t.stride;
+    char* sp = static_cast<char*>(s.p) + k * s.stride;
+    char* xp = static_cast<char*>(x.p) + k * x.stride;
+    double n = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      cx<double> pv[6], tv[6], sv[6], xv[6];
+      SpinorIO<F64>::load_half<1>(pp, vh, i, h, pv);
+      SpinorIO<F64>::load_half<1>(tp, vh, i, h, tv);
+      SpinorIO<F64>::load_half<1>(sp, vh, i, h, sv);
+      SpinorIO<F64>::load_half<1>(xp, vh, i, h, xv);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        xv[c] = {xv[c].re + alpha * pv[c].re, xv[c].im + alpha * pv[c].im};
+        sv[c] = sv[c] - alpha * tv[c];
+        n += norm2(sv[c]);
+      }
+      SpinorIO<F64>::store_half(xp, vh, i, h, xv);
+      SpinorIO<F64>::store_half(sp, vh, i, h, sv);
+    }
+    acc = n;
+  } else if (i < vh) {
     const double alpha = sc[k].alpha;
     cx<T> pv[12], tv[12], sv[12];
     cx<TX> xv[12];

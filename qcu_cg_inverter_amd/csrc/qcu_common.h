@@ -83,9 +83,14 @@ inline Geom make_geom(const int L[4]) {
   return g;
 }
 
-// element counts per site (in units of the precision's storage word)
+// Spinor fields are stored in blocks of kSiteBlock half sites (SpinorIO), so a field
+// spans the half volume rounded up to whole blocks; the sites past vh are never read.
+constexpr int kSiteBlockLog = 6;
+constexpr int kSiteBlock = 1 << kSiteBlockLog;
+__host__ __device__ inline long padded_sites(long vh) { return (vh + kSiteBlock - 1) & ~(long)(kSiteBlock - 1); }
 inline size_t spinor_bytes(int prec, long vh) {
-  return prec == F64 ? (size_t)vh * 192 : prec == F32 ? (size_t)vh * 96 : (size_t)vh * 52;
+  const size_t n = (size_t)padded_sites(vh);
+  return prec == F64 ? n * 192 : prec == F32 ? n * 96 : n * 52;
 }
 inline int link_entries(int recon) { return recon == 12 ? 6 : 9; }
 inline size_t gauge_bytes(int prec, int recon, long vh) {
