@@ -26,10 +26,10 @@ __device__ __forceinline__ void load_link_at(const double2* g, const Geom& geo, 
   t = t < 0 ? t + geo.L[3] : (t >= geo.L[3] ? t - geo.L[3] : t);
   int par = (x + y + z + t) & 1;
   long idx = t * geo.st + z * geo.sz + y * geo.sy + (x >> 1);
-  const double2* b = g + (long)(mu * 2 + par) * 9 * geo.vh + idx;
+  const double2* b = link_block<9>(g, geo.vh, mu * 2 + par, idx);
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
-    double2 q = b[e * geo.vh];
+    double2 q = b[e * kSiteBlock];
     u[e] = {q.x, q.y};
   }
 }

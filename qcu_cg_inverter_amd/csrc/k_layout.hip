@@ -55,7 +55,8 @@ void launch_spinor_to_aos(int prec, Vec in, void* aos, long aos_stride, long vh,
   else hipLaunchKernelGGL(spinor_to_aos_kernel<F16>, grid, dim3(256), 0, st, in, a, aos_stride, vh);
 }
 
-// gauge: thread per (link, mu, parity); the interface link is 9 complex row-major
+// gauge: thread per (link, mu, parity); the interface link is 9 complex row-major; the
+// packed field is written in the site blocks LinkIO reads
 template <int P, int R>
 __global__ void gauge_pack_kernel(const double2* aos, void* out, long vh) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,36 +65,37 @@ __global__ void gauge_pack_kernel(const double2* aos, void* out, long vh) {
   const double2* s = aos + ((long)mp * vh + i) * 9;
   constexpr int NE = R == 12 ? 6 : 9;
   if (P == F64) {
-    double2* d = static_cast<double2*>(out) + (long)mp * NE * vh + i;
+    double2* d = link_block<NE>(static_cast<double2*>(out), vh, mp, i);
 #pragma unroll
-    for (int e = 0; e < NE; ++e) d[e * vh] = s[e];
+    for (int e = 0; e < NE; ++e) d[e * kSiteBlock] = s[e];
   } else if (P == F32) {
     if (R == 12) {
-      float4* d = static_cast<float4*>(out) + (long)mp * 3 * vh + i;
+      float4* d = link_block<3>(static_cast<float4*>(out), vh, mp, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double2 a = s[2 * k], b = s[2 * k + 1];
-        d[k * vh] = float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
+        d[k * kSiteBlock] = float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
       }
     } else {
-      float2* d = static_cast<float2*>(out) + (long)mp * 9 * vh + i;
+      float2* d = link_block<9>(static_cast<float2*>(out), vh, mp, i);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) d[e * vh] = float2{(float)s[e].x, (float)s[e].y};
+      for (int e = 0; e < 9; ++e) d[e * kSiteBlock] = float2{(float)s[e].x, (float)s[e].y};
     }
   } else {
     if (R == 12) {
-      uint2* d = static_cast<uint2*>(out) + (long)mp * 3 * vh + i;
+      uint2* d = link_block<3>(static_cast<uint2*>(out), vh, mp, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double2 a = s[2 * k], b = s[2 * k + 1];
         unsigned lo = (unsigned)f2h((float)a.x) | ((unsigned)f2h((float)a.y) << 16);
         unsigned hi = (unsigned)f2h((float)b.x) | ((unsigned)f2h((float)b.y) << 16);
-        d[k * vh] = uint2{lo, hi};
+        d[k * kSiteBlock] = uint2{lo, hi};
       }
     } else {
-      unsigned* d = static_cast<unsigned*>(out) + (long)mp * 9 * vh + i;
+      unsigned* d = link_block<9>(static_cast<unsigned*>(out), vh, mp, i);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) d[e * vh] = (unsigned)f2h((float)s[e].x) | ((unsigned)f2h((float)s[e].y) << 16);
+      for (int e = 0; e < 9; ++e)
+        d[e * kSiteBlock] = (unsigned)f2h((float)s[e].x) | ((unsigned)f2h((float)s[e].y) << 16);
     }
   }
 }

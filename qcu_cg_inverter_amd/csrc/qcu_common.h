@@ -94,7 +94,7 @@ inline size_t spinor_bytes(int prec, long vh) {
 }
 inline int link_entries(int recon) { return recon == 12 ? 6 : 9; }
 inline size_t gauge_bytes(int prec, int recon, long vh) {
-  size_t cplx = (size_t)8 * link_entries(recon) * vh;  // 4 mu x 2 parities
+  size_t cplx = (size_t)8 * link_entries(recon) * padded_sites(vh);  // 4 mu x 2 parities, site blocks
   return cplx * (prec == F64 ? 16 : prec == F32 ? 8 : 4);
 }
 inline size_t clover_bytes(int prec, long vh) {  // both parities

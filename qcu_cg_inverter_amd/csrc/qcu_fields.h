@@ -300,17 +300,29 @@ __device__ __forceinline__ void recon_row2(cx<T> u[9]) {
 template <int P, int R>
 struct LinkIO;
 
+// Packed links: field f = mu * 2 + parity holds NE words per site in blocks of
+// kSiteBlock sites, like the spinors; word e of site i is at link_block(...)[e * kSiteBlock].
+template <int NE, typename V>
+__device__ __forceinline__ const V* link_block(const V* g, long vh, int f, long i) {
+  return g + (long)f * NE * padded_sites(vh) + (i >> kSiteBlockLog) * (long)(NE * kSiteBlock) +
+         (i & (kSiteBlock - 1));
+}
+template <int NE, typename V>
+__device__ __forceinline__ V* link_block(V* g, long vh, int f, long i) {
+  return g + (long)f * NE * padded_sites(vh) + (i >> kSiteBlockLog) * (long)(NE * kSiteBlock) +
+         (i & (kSiteBlock - 1));
+}
+
 template <int R>
 struct LinkIO<F64, R> {
   using T = double;
   static constexpr int NE = R == 12 ? 6 : 9;
-  // plane base of (mu, parity)
   template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
-    const double2* b = static_cast<const double2*>(g) + (long)(mu * 2 + par) * NE * vh + i;
+    const double2* b = link_block<NE>(static_cast<const double2*>(g), vh, mu * 2 + par, i);
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      double2 q = ldp<NT>(b + e * vh);
+      double2 q = ldp<NT>(b + e * kSiteBlock);
       u[e] = {q.x, q.y};
     }
     if (R == 12) recon_row2(u);
@@ -323,19 +335,19 @@ struct LinkIO<F32, R> {
   template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     if (R == 12) {
-      const float4* b = static_cast<const float4*>(g) + (long)(mu * 2 + par) * 3 * vh + i;
+      const float4* b = link_block<3>(static_cast<const float4*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        float4 q = ldp<NT>(b + k * vh);
+        float4 q = ldp<NT>(b + k * kSiteBlock);
         u[2 * k] = {q.x, q.y};
         u[2 * k + 1] = {q.z, q.w};
       }
       recon_row2(u);
     } else {
-      const float2* b = static_cast<const float2*>(g) + (long)(mu * 2 + par) * 9 * vh + i;
+      const float2* b = link_block<9>(static_cast<const float2*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
-        float2 q = ldp<NT>(b + e * vh);
+        float2 q = ldp<NT>(b + e * kSiteBlock);
         u[e] = {q.x, q.y};
       }
     }
@@ -348,19 +360,19 @@ struct LinkIO<F16, R> {
   template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     if (R == 12) {
-      const uint2* b = static_cast<const uint2*>(g) + (long)(mu * 2 + par) * 3 * vh + i;
+      const uint2* b = link_block<3>(static_cast<const uint2*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        uint2 q = ldp<NT>(b + k * vh);
+        uint2 q = ldp<NT>(b + k * kSiteBlock);
         u[2 * k] = {h2f(q.x & 0xffff), h2f(q.x >> 16)};
         u[2 * k + 1] = {h2f(q.y & 0xffff), h2f(q.y >> 16)};
       }
       recon_row2(u);
     } else {
-      const unsigned* b = static_cast<const unsigned*>(g) + (long)(mu * 2 + par) * 9 * vh + i;
+      const unsigned* b = link_block<9>(static_cast<const unsigned*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
-        unsigned q = ldp<NT>(b + e * vh);
+        unsigned q = ldp<NT>(b + e * kSiteBlock);
         u[e] = {h2f(q & 0xffff), h2f(q >> 16)};
       }
     }
