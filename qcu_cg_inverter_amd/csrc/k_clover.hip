@@ -26,10 +26,11 @@ __device__ __forceinline__ void load_link_at(const double2* g, const Geom& geo, 
   t = t < 0 ? t + geo.L[3] : (t >= geo.L[3] ? t - geo.L[3] : t);
   int par = (x + y + z + t) & 1;
   long idx = t * geo.st + z * geo.sz + y * geo.sy + (x >> 1);
-  const double2* b = link_block<9>(g, geo.vh, mu * 2 + par, idx);
+  const double2* b = link_word<F64, 18>(g, geo.vh, mu * 2 + par, idx);
+  const long ws = link_stride<F64, 18>(geo.vh);
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
-    double2 q = b[e * kSiteBlock];
+    double2 q = b[e * ws];
     u[e] = {q.x, q.y};
   }
 }

@@ -56,7 +56,7 @@ void launch_spinor_to_aos(int prec, Vec in, void* aos, long aos_stride, long vh,
 }
 
 // gauge: thread per (link, mu, parity); the interface link is 9 complex row-major; the
-// packed field is written in the site blocks LinkIO reads
+// packed field is written in the layout LinkIO reads (LinkLayout)
 template <int P, int R>
 __global__ void gauge_pack_kernel(const double2* aos, void* out, long vh) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,26 +64,30 @@ __global__ void gauge_pack_kernel(const double2* aos, void* out, long vh) {
   if (i >= vh) return;
   const double2* s = aos + ((long)mp * vh + i) * 9;
   constexpr int NE = R == 12 ? 6 : 9;
+  static_assert(LinkLayout<F32, 12>::blocked && LinkLayout<F32, 18>::blocked && LinkLayout<F16, 12>::blocked &&
+                    LinkLayout<F16, 18>::blocked,
+                "the FP32 and FP16 stores below step by kSiteBlock");
   if (P == F64) {
-    double2* d = link_block<NE>(static_cast<double2*>(out), vh, mp, i);
+    double2* d = link_word<F64, R>(static_cast<double2*>(out), vh, mp, i);
+    const long ws = link_stride<F64, R>(vh);
 #pragma unroll
-    for (int e = 0; e < NE; ++e) d[e * kSiteBlock] = s[e];
+    for (int e = 0; e < NE; ++e) d[e * ws] = s[e];
   } else if (P == F32) {
     if (R == 12) {
-      float4* d = link_block<3>(static_cast<float4*>(out), vh, mp, i);
+      float4* d = link_word<F32, 12>(static_cast<float4*>(out), vh, mp, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double2 a = s[2 * k], b = s[2 * k + 1];
         d[k * kSiteBlock] = float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
       }
     } else {
-      float2* d = link_block<9>(static_cast<float2*>(out), vh, mp, i);
+      float2* d = link_word<F32, 18>(static_cast<float2*>(out), vh, mp, i);
 #pragma unroll
       for (int e = 0; e < 9; ++e) d[e * kSiteBlock] = float2{(float)s[e].x, (float)s[e].y};
     }
   } else {
     if (R == 12) {
-      uint2* d = link_block<3>(static_cast<uint2*>(out), vh, mp, i);
+      uint2* d = link_word<F16, 12>(static_cast<uint2*>(out), vh, mp, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double2 a = s[2 * k], b = s[2 * k + 1];
@@ -92,7 +96,7 @@ __global__ void gauge_pack_kernel(const double2* aos, void* out, long vh) {
         d[k * kSiteBlock] = uint2{lo, hi};
       }
     } else {
-      unsigned* d = link_block<9>(static_cast<unsigned*>(out), vh, mp, i);
+      unsigned* d = link_word<F16, 18>(static_cast<unsigned*>(out), vh, mp, i);
 #pragma unroll
       for (int e = 0; e < 9; ++e)
         d[e * kSiteBlock] = (unsigned)f2h((float)s[e].x) | ((unsigned)f2h((float)s[e].y) << 16);

@@ -300,17 +300,27 @@ __device__ __forceinline__ void recon_row2(cx<T> u[9]) {
 template <int P, int R>
 struct LinkIO;
 
-// Packed links: field f = mu * 2 + parity holds NE words per site in blocks of
-// kSiteBlock sites, like the spinors; word e of site i is at link_block(...)[e * kSiteBlock].
-template <int NE, typename V>
-__device__ __forceinline__ const V* link_block(const V* g, long vh, int f, long i) {
-  return g + (long)f * NE * padded_sites(vh) + (i >> kSiteBlockLog) * (long)(NE * kSiteBlock) +
-         (i & (kSiteBlock - 1));
+// Packed links: field f = mu * 2 + parity holds NE words per site, either in blocks of
+// kSiteBlock sites like the spinors (word e of site i at link_word(...)[e *
+// link_stride]) or, for FP64 with all 18 reals, in one plane per word.  Measured on
+// MI355X, blocks made the FP32 and FP16 hops and the FP64 recon-12 hop 2-18 % faster
+// and the FP64 recon-18 hop 4-11 % slower, so each form keeps the faster layout.
+template <int P, int R>
+struct LinkLayout {
+  static constexpr bool blocked = !(P == F64 && R == 18);
+  static constexpr int NE = P == F64 ? (R == 12 ? 6 : 9) : (R == 12 ? 3 : 9);  // words per link
+};
+template <int P, int R>
+__host__ __device__ inline long link_stride(long vh) {
+  return LinkLayout<P, R>::blocked ? (long)kSiteBlock : vh;
 }
-template <int NE, typename V>
-__device__ __forceinline__ V* link_block(V* g, long vh, int f, long i) {
-  return g + (long)f * NE * padded_sites(vh) + (i >> kSiteBlockLog) * (long)(NE * kSiteBlock) +
-         (i & (kSiteBlock - 1));
+template <int P, int R, typename V>
+__device__ __forceinline__ V* link_word(V* g, long vh, int f, long i) {
+  constexpr int NE = LinkLayout<P, R>::NE;
+  if (LinkLayout<P, R>::blocked)
+    return g + (long)f * NE * padded_sites(vh) + (i >> kSiteBlockLog) * (long)(NE * kSiteBlock) +
+           (i & (kSiteBlock - 1));
+  return g + (long)f * NE * padded_sites(vh) + i;
 }
 
 template <int R>
@@ -319,10 +329,11 @@ struct LinkIO<F64, R> {
   static constexpr int NE = R == 12 ? 6 : 9;
   template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
-    const double2* b = link_block<NE>(static_cast<const double2*>(g), vh, mu * 2 + par, i);
+    const double2* b = link_word<F64, R>(static_cast<const double2*>(g), vh, mu * 2 + par, i);
+    const long ws = link_stride<F64, R>(vh);
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      double2 q = ldp<NT>(b + e * kSiteBlock);
+      double2 q = ldp<NT>(b + e * ws);
       u[e] = {q.x, q.y};
     }
     if (R == 12) recon_row2(u);
@@ -335,7 +346,7 @@ struct LinkIO<F32, R> {
   template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     if (R == 12) {
-      const float4* b = link_block<3>(static_cast<const float4*>(g), vh, mu * 2 + par, i);
+      const float4* b = link_word<F32, R>(static_cast<const float4*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         float4 q = ldp<NT>(b + k * kSiteBlock);
@@ -344,7 +355,7 @@ struct LinkIO<F32, R> {
       }
       recon_row2(u);
     } else {
-      const float2* b = link_block<9>(static_cast<const float2*>(g), vh, mu * 2 + par, i);
+      const float2* b = link_word<F32, R>(static_cast<const float2*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
         float2 q = ldp<NT>(b + e * kSiteBlock);
@@ -360,7 +371,7 @@ struct LinkIO<F16, R> {
   template <int NT = 1>
   static __device__ __forceinline__ void load(const void* g, long vh, int mu, int par, long i, cx<T> u[9]) {
     if (R == 12) {
-      const uint2* b = link_block<3>(static_cast<const uint2*>(g), vh, mu * 2 + par, i);
+      const uint2* b = link_word<F16, R>(static_cast<const uint2*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         uint2 q = ldp<NT>(b + k * kSiteBlock);
@@ -369,7 +380,7 @@ struct LinkIO<F16, R> {
       }
       recon_row2(u);
     } else {
-      const unsigned* b = link_block<9>(static_cast<const unsigned*>(g), vh, mu * 2 + par, i);
+      const unsigned* b = link_word<F16, R>(static_cast<const unsigned*>(g), vh, mu * 2 + par, i);
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
         unsigned q = ldp<NT>(b + e * kSiteBlock);
