@@ -204,9 +204,9 @@ __device__ __forceinline__ void pack_block(const zd m[36], double2* out, long vh
       r[k + 1] = m[i * 6 + j].im;
       k += 2;
     }
-  double2* o = out + (long)b * 18 * vh + site;
+  double2* o = site_block<36>(out, site) + b * 18 * kSiteBlock;
 #pragma unroll
-  for (int q = 0; q < 18; ++q) o[q * vh] = double2{r[2 * q], r[2 * q + 1]};
+  for (int q = 0; q < 18; ++q) o[q * kSiteBlock] = double2{r[2 * q], r[2 * q + 1]};
 }
 
 // thread per (site, chiral block b)
@@ -251,26 +251,28 @@ __global__ void clover_assemble_kernel(const double2* G, double2* A, double2* Ai
   pack_block(inv, Ainv, vh, site, b);
 }
 
-// packed F64 -> F32 (float4 [18][vh]) / F16 (uint4 [9][vh]); one parity
+// packed F64 -> F32 (18 float4 words per site) / F16 (9 uint4 words); one parity
 template <int P>
 __global__ void clover_convert_kernel(const double2* src, void* dst, long vh) {
   long site = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (site >= vh) return;
   float r[72];
+  const double2* sp = site_block<36>(src, site);
 #pragma unroll
   for (int k = 0; k < 36; ++k) {
-    double2 q = src[k * vh + site];
+    double2 q = sp[k * kSiteBlock];
     r[2 * k] = (float)q.x;
     r[2 * k + 1] = (float)q.y;
   }
   if (P == F32) {
-    float4* o = static_cast<float4*>(dst) + site;
+    float4* o = site_block<18>(static_cast<float4*>(dst), site);
 #pragma unroll
-    for (int k = 0; k < 18; ++k) o[k * vh] = float4{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
+    for (int k = 0; k < 18; ++k)
+      o[k * kSiteBlock] = float4{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
   } else {
-    uint4* o = static_cast<uint4*>(dst) + site;
+    uint4* o = site_block<9>(static_cast<uint4*>(dst), site);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) o[k * vh] = pack8(r + 8 * k);
+    for (int k = 0; k < 9; ++k) o[k * kSiteBlock] = pack8(r + 8 * k);
   }
 }
 
@@ -279,9 +281,10 @@ __global__ void clover_unpack_kernel(const double2* src, double2* out, long vh) 
   long site = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (site >= vh) return;
   double a[72];
+  const double2* sp = site_block<36>(src, site);
 #pragma unroll
   for (int k = 0; k < 36; ++k) {
-    double2 q = src[k * vh + site];
+    double2 q = sp[k * kSiteBlock];
     a[2 * k] = q.x;
     a[2 * k + 1] = q.y;
   }

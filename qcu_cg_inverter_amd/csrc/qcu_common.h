@@ -97,8 +97,8 @@ inline size_t gauge_bytes(int prec, int recon, long vh) {
   size_t cplx = (size_t)8 * link_entries(recon) * padded_sites(vh);  // 4 mu x 2 parities, site blocks
   return cplx * (prec == F64 ? 16 : prec == F32 ? 8 : 4);
 }
-inline size_t clover_bytes(int prec, long vh) {  // both parities
-  return (size_t)2 * vh * 72 * (prec == F64 ? 8 : prec == F32 ? 4 : 2);
+inline size_t clover_bytes(int prec, long vh) {  // both parities, site blocks
+  return (size_t)2 * padded_sites(vh) * 72 * (prec == F64 ? 8 : prec == F32 ? 4 : 2);
 }
 
 // Epilogue of the hopping kernel: what is written for out(parity p), where
