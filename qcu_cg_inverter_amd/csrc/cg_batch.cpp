@@ -190,10 +190,10 @@ void CgBatch::launch_chunk() {
   auto iteration = [&]() {
     mhat(prec_, 1, S_, R_, TO_, 0, n, part_rr_);
     launch_cg_beta(part_rr_, nw_, dsc_, n, st_);
-    launch_cg_xpay(prec_, R_, P_, dsc_, vh_, n, st_);
+    launch_cg_xpay(prec_, xprec_, R_, P_, X_, dsc_, vh_, n, st_);
     mhat(prec_, 0, P_, T_, TO_, 0, n, part_tt_);
     launch_cg_alpha(part_tt_, nw_, dsc_, n, st_);
-    launch_cg_update(prec_, xprec_, X_, P_, T_, S_, dsc_, part_ss_, vh_, n, st_);
+    launch_cg_update(prec_, T_, S_, dsc_, part_ss_, vh_, n, st_);
     launch_cg_ss(part_ss_, nw_, dsc_, n, st_);
   };
   if (!sp_.graph) {
@@ -212,8 +212,18 @@ void CgBatch::launch_chunk() {
   QHIP(hipGraphLaunch(gr.second, st_));
 }
 
+// The iterate update x += alpha p of a chunk's last iteration is applied by the next
+// iteration's p update; before the host reads x it applies it here (with the host's
+// copy of alpha, which it then clears, so the device does not apply it again).
+void CgBatch::flush_x(int r) {
+  CgScal& c = hsc_[r];
+  if (c.alpha != 0.0) launch_cg_xflush(prec_, xprec_, at(X_, r), at(P_, r), c.alpha, vh_, st_);
+  c.alpha = 0.0;
+}
+
 // s = b^ - Mhat x in FP64 for slot r (the low-precision iterate folded into x first)
 double CgBatch::reliable_update(int r) {
+  flush_x(r);
   if (prec_ != F64) {
     launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
     launch_zero(xprec_, at(X_, r), vh_, 1, st_);
@@ -292,6 +302,7 @@ std::vector<double> CgBatch::reliable_updates(int lo, int hi, const std::vector<
   }
   for (int r = lo; r <= hi; ++r) {
     if (!upd[r]) continue;
+    flush_x(r);
     launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
     launch_zero(xprec_, at(X_, r), vh_, 1, st_);
   }
@@ -329,6 +340,7 @@ const void* CgBatch::source_aos(const Slot& sl) {
 void CgBatch::finish(int r, bool breakdown) {
   Slot& sl = slots_[r];
   const double kap = sl.spec.kappa;
+  flush_x(r);
   if (prec_ != F64) {
     launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
     launch_zero(xprec_, at(X_, r), vh_, 1, st_);

@@ -64,6 +64,7 @@ class CgBatch {
   };
   void check_operator(const SlotSpec& s) const;
   void mhat(int prec, int dag, Vec in, Vec out, Vec tmp, int r0, int n, double* partials);
+  void flush_x(int r);
   double reliable_update(int r);
   std::vector<double> reliable_updates(int lo, int hi, const std::vector<char>& upd);
   void finish(int r, bool breakdown);

@@ -59,23 +59,40 @@ __global__ void norm2_kernel(Vec v, long vh, double* part) {
 // chains are combined in a fixed order, then a fixed LDS tree; the order depends on
 // nw only, so the sum of a right-hand side never depends on the batch
 constexpr int kSumThreads = 1024;
+// Each thread keeps 16 loads in flight (16 chains, the tail as predicated loads rather
+// than a dependent loop), each wave folds its 64 values with lane shuffles, and the 16
+// wave sums are added by one thread: a fixed order that depends on nw only.
+constexpr int kSumChains = 16;
 __device__ __forceinline__ double block_sum_partials(const double* part, long nw) {
-  __shared__ double red[kSumThreads];
-  double c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  __shared__ double red[kSumThreads / 64];
+  double c[kSumChains];
+#pragma unroll
+  for (int k = 0; k < kSumChains; ++k) c[k] = 0.0;
   long w = threadIdx.x;
-  for (; w + 7 * kSumThreads < nw; w += 8 * kSumThreads) {
+  for (; w + (kSumChains - 1) * kSumThreads < nw; w += kSumChains * kSumThreads) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] += part[w + k * kSumThreads];
+    for (int k = 0; k < kSumChains; ++k) c[k] += part[w + k * kSumThreads];
   }
-  for (; w < nw; w += kSumThreads) c[0] += part[w];
-  red[threadIdx.x] = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
+#pragma unroll
+  for (int k = 0; k < kSumChains - 1; ++k) {
+    const long j = w + (long)k * kSumThreads;
+    if (j < nw) c[k] += part[j];
+  }
+#pragma unroll
+  for (int h = kSumChains / 2; h >= 1; h >>= 1)
+#pragma unroll
+    for (int k = 0; k < h; ++k) c[k] += c[k + h];
+  double v = c[0];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
 #pragma unroll
-  for (int off = kSumThreads / 2; off >= 1; off >>= 1) {
-    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
+    for (int k = 0; k < kSumThreads / 64; ++k) s += red[k];
   }
-  return red[0];
+  return s;  // valid in thread 0
 }
 
 __global__ void __launch_bounds__(kSumThreads) sum_partials_kernel(const double* part, long nw, double* out) {
@@ -158,86 +175,123 @@ __global__ void __launch_bounds__(kSumThreads) cg_ss_kernel(const double* part, 
   }
 }
 
-template <int P>
-__global__ void cg_xpay_kernel(Vec r, Vec p, const CgScal* sc, long vh) {
+// x += alpha_k p_k of the previous iteration (still pending: alpha is that iteration's,
+// p not yet overwritten), then p = r + beta p.  Folding the iterate update in here
+// saves one read of p per iteration against doing it in the s update; the host
+// applies a pending update itself (cg_xflush) before it reads x.
+template <int P, int PX>
+__global__ void cg_xpay_kernel(Vec r, Vec p, Vec x, const CgScal* sc, long vh) {
   using T = typename Real<P>::T;
+  using TX = typename Real<PX>::T;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y;
   if (i >= vh || !sc[k].active) return;
+  const double alpha = sc[k].alpha;
   const T beta = T(sc[k].beta);
-  cx<T> rv[12], pv[12];
-  SpinorIO<P>::load(static_cast<const char*>(r.p) + k * r.stride, vh, i, rv);
   char* pp = static_cast<char*>(p.p) + k * p.stride;
+  char* xp = static_cast<char*>(x.p) + k * x.stride;
+  const char* rp = static_cast<const char*>(r.p) + k * r.stride;
+  if (P == F64 && PX == F64) {
+    // FP64 in halves of 6 components (p, r and x at once would spill)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      cx<double> pv[6], rv[6];
+      if (alpha != 0.0 || beta != T(0)) SpinorIO<F64>::load_half<1>(pp, vh, i, h, pv);
+      if (alpha != 0.0) {
+        cx<double> xv[6];
+        SpinorIO<F64>::load_half<1>(xp, vh, i, h, xv);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) xv[c] = {xv[c].re + alpha * pv[c].re, xv[c].im + alpha * pv[c].im};
+        SpinorIO<F64>::store_half(xp, vh, i, h, xv);
+      }
+      SpinorIO<F64>::load_half<1>(rp, vh, i, h, rv);
+      if (beta != T(0)) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) rv[c] = rv[c] + (double)beta * pv[c];
+      }
+      SpinorIO<F64>::store_half(pp, vh, i, h, rv);
+    }
+    return;
+  }
+  cx<T> pv[12];
+  if (alpha != 0.0 || beta != T(0)) SpinorIO<P>::load(pp, vh, i, pv);
+  if (alpha != 0.0) {
+    const TX ax = TX(alpha);
+    cx<TX> xv[12];
+    SpinorIO<PX>::load(xp, vh, i, xv);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) xv[c] = {xv[c].re + ax * TX(pv[c].re), xv[c].im + ax * TX(pv[c].im)};
+    SpinorIO<PX>::store(xp, vh, i, xv);
+  }
+  cx<T> rv[12];
+  SpinorIO<P>::load(rp, vh, i, rv);
   if (beta == T(0)) {
-    // restart (first iteration or converged): p = r without reading p, whose old
-    // contents need not be finite
+    // restart (first iteration, after a restart, or converged): p = r
     SpinorIO<P>::store(pp, vh, i, rv);
     return;
   }
-  SpinorIO<P>::load(pp, vh, i, pv);
 #pragma unroll
   for (int c = 0; c < 12; ++c) pv[c] = rv[c] + beta * pv[c];
   SpinorIO<P>::store(pp, vh, i, pv);
 }
 
-template <int P, int PX>
-__global__ void cg_update_kernel(Vec x, Vec p, Vec t, Vec s, const CgScal* sc, double* part, long vh) {
+// s -= alpha t, with the 16-site partials of |s|^2
+template <int P>
+__global__ void cg_update_kernel(Vec t, Vec s, const CgScal* sc, double* part, long vh) {
   using T = typename Real<P>::T;
-  using TX = typename Real<PX>::T;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y;
   if (!sc[k].active) return;
   double acc = 0.0;
-  if (P == F64 && PX == F64 && i < vh) {
-    // FP64: the four spinors of a site in two halves of 6 components, which fits the
-    // registers (all 48 complex at once spill); same operations in the same order
-    const double alpha = sc[k].alpha;
-    const char* pp = static_cast<const char*>(p.p) + k * p.stride;
-    const char* tp = static_cast<const char*>(t.p) + k * t.stride;
+  if (i < vh) {
+    const T a = T(sc[k].alpha);
     char* sp = static_cast<char*>(s.p) + k * s.stride;
-    char* xp = static_cast<char*>(x.p) + k * x.stride;
-    double n = 0.0;
+    const char* tp = static_cast<const char*>(t.p) + k * t.stride;
+    T n = T(0);
+    if (P == F64) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      cx<double> pv[6], tv[6], sv[6], xv[6];
-      SpinorIO<F64>::load_half<1>(pp, vh, i, h, pv);
-      SpinorIO<F64>::load_half<1>(tp, vh, i, h, tv);
-      SpinorIO<F64>::load_half<1>(sp, vh, i, h, sv);
-      SpinorIO<F64>::load_half<1>(xp, vh, i, h, xv);
+      for (int h = 0; h < 2; ++h) {
+        cx<double> tv[6], sv[6];
+        SpinorIO<F64>::load_half<1>(tp, vh, i, h, tv);
+        SpinorIO<F64>::load_half<1>(sp, vh, i, h, sv);
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        xv[c] = {xv[c].re + alpha * pv[c].re, xv[c].im + alpha * pv[c].im};
-        sv[c] = sv[c] - alpha * tv[c];
+        for (int c = 0; c < 6; ++c) {
+          sv[c] = sv[c] - (double)a * tv[c];
+          n += T(norm2(sv[c]));
+        }
+        SpinorIO<F64>::store_half(sp, vh, i, h, sv);
+      }
+    } else {
+      cx<T> tv[12], sv[12];
+      SpinorIO<P>::load(tp, vh, i, tv);
+      SpinorIO<P>::load(sp, vh, i, sv);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) {
+        sv[c] = sv[c] - a * tv[c];
         n += norm2(sv[c]);
       }
-      SpinorIO<F64>::store_half(xp, vh, i, h, xv);
-      SpinorIO<F64>::store_half(sp, vh, i, h, sv);
+      SpinorIO<P>::store(sp, vh, i, sv);
     }
-    acc = n;
-  } else if (i < vh) {
-    const double alpha = sc[k].alpha;
-    cx<T> pv[12], tv[12], sv[12];
-    cx<TX> xv[12];
-    SpinorIO<P>::load(static_cast<const char*>(p.p) + k * p.stride, vh, i, pv);
-    SpinorIO<P>::load(static_cast<const char*>(t.p) + k * t.stride, vh, i, tv);
-    char* sp = static_cast<char*>(s.p) + k * s.stride;
-    char* xp = static_cast<char*>(x.p) + k * x.stride;
-    SpinorIO<P>::load(sp, vh, i, sv);
-    SpinorIO<PX>::load(xp, vh, i, xv);
-    const T a = T(alpha);
-    const TX ax = TX(alpha);
-    T n = T(0);
-#pragma unroll
-    for (int c = 0; c < 12; ++c) {
-      xv[c] = {xv[c].re + ax * TX(pv[c].re), xv[c].im + ax * TX(pv[c].im)};
-      sv[c] = sv[c] - a * tv[c];
-      n += norm2(sv[c]);
-    }
-    SpinorIO<PX>::store(xp, vh, i, xv);
-    SpinorIO<P>::store(sp, vh, i, sv);
     acc = (double)n;
   }
   put_partial(part, vh, k, i, acc);
+}
+
+// x += alpha p for one slot: a pending iterate update applied by the host
+template <int P, int PX>
+__global__ void cg_xflush_kernel(Vec x, Vec p, double alpha, long vh) {
+  using T = typename Real<P>::T;
+  using TX = typename Real<PX>::T;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cx<T> pv[12];
+  cx<TX> xv[12];
+  SpinorIO<P>::load(p.p, vh, i, pv);
+  SpinorIO<PX>::load(x.p, vh, i, xv);
+  const TX ax = TX(alpha);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) xv[c] = {xv[c].re + ax * TX(pv[c].re), xv[c].im + ax * TX(pv[c].im)};
+  SpinorIO<PX>::store(x.p, vh, i, xv);
 }
 
 // plain streaming copy, the bandwidth and PMC-counter reference kernel: one 16-byte
@@ -286,17 +340,24 @@ void launch_cg_alpha(const double* part, long nw, CgScal* sc, int nrhs, hipStrea
 void launch_cg_ss(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t st) {
   hipLaunchKernelGGL(cg_ss_kernel, dim3(nrhs), dim3(kSumThreads), 0, st, part, nw, sc);
 }
-void launch_cg_xpay(int prec, Vec r, Vec p, const CgScal* sc, long vh, int nrhs, hipStream_t st) {
-  QCU_PSW(prec, cg_xpay_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, r, p, sc, vh);
-}
-void launch_cg_update(int prec, int xprec, Vec x, Vec p, Vec t, Vec s, const CgScal* sc, double* part, long vh,
-                      int nrhs, hipStream_t st) {
+void launch_cg_xpay(int prec, int xprec, Vec r, Vec p, Vec x, const CgScal* sc, long vh, int nrhs, hipStream_t st) {
   dim3 g(nblk(vh, 256), nrhs), b(256);
-  if (prec == F64) hipLaunchKernelGGL((cg_update_kernel<F64, F64>), g, b, 0, st, x, p, t, s, sc, part, vh);
-  else if (prec == F32 && xprec == F64) hipLaunchKernelGGL((cg_update_kernel<F32, F64>), g, b, 0, st, x, p, t, s, sc, part, vh);
-  else if (prec == F32) hipLaunchKernelGGL((cg_update_kernel<F32, F32>), g, b, 0, st, x, p, t, s, sc, part, vh);
-  else if (xprec == F64) hipLaunchKernelGGL((cg_update_kernel<F16, F64>), g, b, 0, st, x, p, t, s, sc, part, vh);
-  else hipLaunchKernelGGL((cg_update_kernel<F16, F32>), g, b, 0, st, x, p, t, s, sc, part, vh);
+  if (prec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F64, F64>), g, b, 0, st, r, p, x, sc, vh);
+  else if (prec == F32 && xprec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F32, F64>), g, b, 0, st, r, p, x, sc, vh);
+  else if (prec == F32) hipLaunchKernelGGL((cg_xpay_kernel<F32, F32>), g, b, 0, st, r, p, x, sc, vh);
+  else if (xprec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F16, F64>), g, b, 0, st, r, p, x, sc, vh);
+  else hipLaunchKernelGGL((cg_xpay_kernel<F16, F32>), g, b, 0, st, r, p, x, sc, vh);
+}
+void launch_cg_xflush(int prec, int xprec, Vec x, Vec p, double alpha, long vh, hipStream_t st) {
+  dim3 g(nblk(vh, 256)), b(256);
+  if (prec == F64) hipLaunchKernelGGL((cg_xflush_kernel<F64, F64>), g, b, 0, st, x, p, alpha, vh);
+  else if (prec == F32 && xprec == F64) hipLaunchKernelGGL((cg_xflush_kernel<F32, F64>), g, b, 0, st, x, p, alpha, vh);
+  else if (prec == F32) hipLaunchKernelGGL((cg_xflush_kernel<F32, F32>), g, b, 0, st, x, p, alpha, vh);
+  else if (xprec == F64) hipLaunchKernelGGL((cg_xflush_kernel<F16, F64>), g, b, 0, st, x, p, alpha, vh);
+  else hipLaunchKernelGGL((cg_xflush_kernel<F16, F32>), g, b, 0, st, x, p, alpha, vh);
+}
+void launch_cg_update(int prec, Vec t, Vec s, const CgScal* sc, double* part, long vh, int nrhs, hipStream_t st) {
+  QCU_PSW(prec, cg_update_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, t, s, sc, part, vh);
 }
 
 }  // namespace qcu

@@ -60,11 +60,13 @@ void launch_clover_apply(int prec, const void* clov, Vec in, Vec out, long vh, i
 void launch_cg_beta(const double* part_rr, long nw, CgScal* sc, int nrhs, hipStream_t st);
 void launch_cg_alpha(const double* part_tt, long nw, CgScal* sc, int nrhs, hipStream_t st);
 void launch_cg_ss(const double* part_ss, long nw, CgScal* sc, int nrhs, hipStream_t st);
+// x += alpha p with the alpha and p of the previous iteration (pending until here), then
 // p = r + beta p
-void launch_cg_xpay(int prec, Vec r, Vec p, const CgScal* sc, long vh, int nrhs, hipStream_t st);
-// x += alpha p ; s -= alpha t ; partials of |s|^2
-void launch_cg_update(int prec, int xprec, Vec x, Vec p, Vec t, Vec s, const CgScal* sc, double* part_ss, long vh,
-                      int nrhs, hipStream_t st);
+void launch_cg_xpay(int prec, int xprec, Vec r, Vec p, Vec x, const CgScal* sc, long vh, int nrhs, hipStream_t st);
+// s -= alpha t ; partials of |s|^2
+void launch_cg_update(int prec, Vec t, Vec s, const CgScal* sc, double* part_ss, long vh, int nrhs, hipStream_t st);
+// x += alpha p for one right-hand side (a pending iterate update, applied by the host)
+void launch_cg_xflush(int prec, int xprec, Vec x, Vec p, double alpha, long vh, hipStream_t st);
 
 // --------------------------------------------------------------- clover -----
 // G[pl] = Q - Q^dag for the 6 planes at every site of `parity`: double2 [6][9][Vh]

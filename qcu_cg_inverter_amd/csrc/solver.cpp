@@ -11,7 +11,9 @@
 // and CGLS (CG on Mhat^dag Mhat x = Mhat^dag b^ that carries s = b^ - Mhat x):
 //   r = Mhat^dag s ; beta = |r|^2/gamma ; gamma = |r|^2 ; p = r + beta p
 //   t = Mhat p ; alpha = gamma/|t|^2 ; x += alpha p ; s -= alpha t
-// so the stopping test |s| <= tol |b| is on the residual of the system itself.
+// so the stopping test |s| <= tol |b| is on the residual of the system itself.  The
+// step x += alpha p is carried out by the next iteration's p update, which reads p
+// anyway (launch order: r, beta, [x += alpha p, p = r + beta p], t, alpha, s).
 // Mixed precision: the loop runs in FP32 or FP16 (+ per-site norm); when |s| drops
 // below delta times its value at the last update (or reaches the target) the FP32
 // iterate is added to the FP64 solution and s is recomputed in FP64 (reliable
