@@ -311,6 +311,11 @@ def cpu_baseline(seconds_target=15.0):
             break
     vh = orc.half_volume(L)
     gbs = BYTES_PER_SITE[("f64", 18)] * vh * n / el / 1e9
+    # the GPU's 16^3x32 solve on the same inputs: Wilson kappa 0.124, antiperiodic t,
+    # FP64 even-odd CG to 1e-10, all threads
+    t0 = time.perf_counter()
+    _, st2 = orc.cg_eo(v, g, L, 0.124, tbc=-1.0, recon=18, tol=1e-10)
+    tcg2 = time.perf_counter() - t0
     # the same on one core, and on the 24^3x48 lattice with all of them
     orc.set_threads(1)
     n1, t0 = 0, time.perf_counter()
@@ -347,7 +352,9 @@ def cpu_baseline(seconds_target=15.0):
                                 "ms_per_apply": round(1e3 * el3 / n3, 3), "applications": n3, "cores": threads},
             "host_cpu": platform.processor() or platform.machine(),
             "cg_config1_8x8x8x8": {"seconds": round(tcg, 4), "iterations": st.iterations,
-                                   "true_residual": st.true_residual}}
+                                   "true_residual": st.true_residual},
+            "cg_16x16x16x32_wilson": {"seconds": round(tcg2, 3), "iterations": st2.iterations,
+                                      "true_residual": st2.true_residual, "cores": threads}}
 
 
 def config1_gpu(q):
