@@ -54,14 +54,12 @@ __global__ void norm2_kernel(Vec v, long vh, double* part) {
   put_partial(part, vh, r, i, s);
 }
 
-// one block of kSumThreads threads per right-hand side: every thread adds its
-// strided partials in eight independent chains (so eight loads are in flight), the
-// chains are combined in a fixed order, then a fixed LDS tree; the order depends on
-// nw only, so the sum of a right-hand side never depends on the batch
+// One block of kSumThreads threads per right-hand side.  Every thread adds its strided
+// partials in 16 independent chains (16 loads in flight; the tail as predicated loads
+// rather than a dependent loop), the chains are combined in a fixed tree, each wave
+// folds its 64 values with lane shuffles, and one thread adds the 16 wave sums.  The
+// order depends on nw only, so the sum of a right-hand side never depends on the batch.
 constexpr int kSumThreads = 1024;
-// Each thread keeps 16 loads in flight (16 chains, the tail as predicated loads rather
-// than a dependent loop), each wave folds its 64 values with lane shuffles, and the 16
-// wave sums are added by one thread: a fixed order that depends on nw only.
 constexpr int kSumChains = 16;
 __device__ __forceinline__ double block_sum_partials(const double* part, long nw) {
   __shared__ double red[kSumThreads / 64];
