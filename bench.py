@@ -157,6 +157,14 @@ def serving_section(q, n_requests=240, seed=2024):
         ids.append(submit(server, r))
         if (k + 1) % 60 == 0:
             log(f"serving: {k + 1} submitted at {time.perf_counter() - t0:.1f} s")
+    last, pending = time.perf_counter(), list(ids)
+    while pending:  # drain, with a progress line every 10 s
+        pending = [i for i in pending if not server.poll(i).done]
+        if pending and time.perf_counter() - last > 10.0:
+            log(f"serving: {len(ids) - len(pending)} of {len(ids)} done at {time.perf_counter() - t0:.1f} s")
+            last = time.perf_counter()
+        if pending:
+            time.sleep(0.2)
     server.drain()
     tend = time.perf_counter() - t0
     sts = [server.poll(i) for i in ids]

@@ -38,6 +38,8 @@ struct Request {
   int max_iter_seen = 0;
   double worst_res = 0.0;
   Clock::time_point submit, first_admit, done_at;
+  Clock::time_point stalled_since;  // first pass that found no memory for it
+  bool stalled = false;
   bool started = false, done = false;
 };
 
@@ -236,9 +238,12 @@ bool Server::round() {
     }
   }
   // 2. admission in arrival order.  A request whose batch is mid-chunk keeps its place
-  //    while later ones go ahead.  Once one does not fit the memory budget, the ones
-  //    behind it may still fill free slots of existing batches, but nothing new is
-  //    allocated until it is in, so it is not starved
+  //    while later ones go ahead.  Once one does not fit the memory budget, nothing new
+  //    is allocated until it is in, and the ones behind it may still fill free slots of
+  //    existing batches -- but only for kStallGrace: after that nothing behind it is
+  //    admitted, so the batches it waits on drain and give their memory back (with a
+  //    steady stream they would otherwise stay busy and starve it)
+  constexpr double kStallGrace = 0.5;  // seconds
   std::vector<uint64_t> ids;
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -246,6 +251,7 @@ bool Server::round() {
   }
   std::vector<uint64_t> admitted_all;
   bool allow_alloc = true;
+  const Clock::time_point now = Clock::now();
   for (uint64_t id : ids) {
     Request* r;
     {
@@ -255,8 +261,16 @@ bool Server::round() {
     const int before = r->admitted;
     const Admit a = admit_one(id, *r, allow_alloc);
     progress |= r->admitted != before;
-    if (a == ADMIT_ALL) admitted_all.push_back(id);
-    else if (a == ADMIT_NO_MEMORY) allow_alloc = false;
+    if (a == ADMIT_ALL) {
+      admitted_all.push_back(id);
+    } else if (a == ADMIT_NO_MEMORY) {
+      allow_alloc = false;
+      if (!r->stalled) {
+        r->stalled = true;
+        r->stalled_since = now;
+      }
+      if (std::chrono::duration<double>(now - r->stalled_since).count() > kStallGrace) break;
+    }
   }
   if (!admitted_all.empty()) {
     std::lock_guard<std::mutex> lk(mu);
