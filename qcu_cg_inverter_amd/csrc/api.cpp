@@ -124,7 +124,8 @@ size_t GaugeField::bytes() const {
   for (auto& a : packed)
     for (auto& b : a) s += b.n;
   for (auto& kv : clovers)
-    for (int p = 0; p < 3; ++p) s += kv.second->A[p].n + kv.second->Ainv[p].n;
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 2; ++q) s += kv.second->A[p][q].n + kv.second->Ainv[p][q].n;
   return s;
 }
 
@@ -138,41 +139,48 @@ const void* GaugeField::get(int prec, int recon) {
   return b.p;
 }
 
-CloverField* GaugeField::clover(double ck, int prec) {
+CloverField* GaugeField::clover(double ck, int prec, bool full_inverse) {
   auto it = clovers.find(ck);
   CloverField* cf;
   if (it == clovers.end()) {
     std::unique_ptr<CloverField> c(new CloverField);
     c->ck = ck;
     c->vh = geo.vh;
-    c->A[F64].alloc(clover_bytes(F64, geo.vh));
-    c->Ainv[F64].alloc(clover_bytes(F64, geo.vh));
-    const void* g18 = get(F64, 18);
-    DevBuf* G = ctx().buf("clover_G", (size_t)6 * 9 * 16 * geo.vh);
-    DevBuf* nb = ctx().buf("clover_nbad", sizeof(int));
-    QHIP(hipMemsetAsync(nb->p, 0, sizeof(int), stream()));
-    for (int p = 0; p < 2; ++p) {
-      launch_clover_leaves(g18, G->p, geo, p, stream());
-      launch_clover_assemble(G->p, const_cast<void*>(c->a(F64, p)), const_cast<void*>(c->ainv(F64, p)), geo.vh, ck,
-                             static_cast<int*>(nb->p), stream());
-    }
-    QHIP(hipGetLastError());
-    int nbad = 0;
-    QHIP(hipMemcpyAsync(&nbad, nb->p, sizeof(int), hipMemcpyDeviceToHost, stream()));
-    QHIP(hipStreamSynchronize(stream()));
-    c->nbad = nbad;
     cf = c.get();
     clovers[ck] = std::move(c);
   } else {
     cf = it->second.get();
   }
-  if (prec != F64 && !cf->A[prec].p) {
-    cf->A[prec].alloc(clover_bytes(prec, geo.vh));
-    cf->Ainv[prec].alloc(clover_bytes(prec, geo.vh));
+  const size_t half64 = clover_bytes(F64, geo.vh) / 2;
+  // FP64 parts: A on both parities, A^-1 on the odd one, A^-1 on the even one if asked
+  const bool need_a = !cf->A[F64][0].p, need_ie = full_inverse && !cf->Ainv[F64][0].p;
+  if (need_a || need_ie) {
+    const void* g18 = get(F64, 18);
+    DevBuf* G = ctx().buf("clover_G", (size_t)6 * 9 * 16 * geo.vh);
+    DevBuf* nb = ctx().buf("clover_nbad", sizeof(int));
+    QHIP(hipMemsetAsync(nb->p, 0, sizeof(int), stream()));
     for (int p = 0; p < 2; ++p) {
-      launch_clover_convert(prec, cf->a(F64, p), const_cast<void*>(cf->a(prec, p)), geo.vh, stream());
-      launch_clover_convert(prec, cf->ainv(F64, p), const_cast<void*>(cf->ainv(prec, p)), geo.vh, stream());
+      const bool want_a = need_a, want_i = p == 1 ? need_a : (need_ie || (need_a && full_inverse));
+      if (!want_a && !want_i) continue;
+      if (want_a) cf->A[F64][p].alloc(half64);
+      if (want_i) cf->Ainv[F64][p].alloc(half64);
+      launch_clover_leaves(g18, G->p, geo, p, stream());
+      launch_clover_assemble(G->p, want_a ? cf->A[F64][p].p : nullptr, want_i ? cf->Ainv[F64][p].p : nullptr,
+                             geo.vh, ck, static_cast<int*>(nb->p), stream());
     }
+    QHIP(hipGetLastError());
+    int nbad = 0;
+    QHIP(hipMemcpyAsync(&nbad, nb->p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    QHIP(hipStreamSynchronize(stream()));
+    cf->nbad += nbad;
+  }
+  // inner-precision parts: A_ee and A^-1_oo
+  if (prec != F64 && !cf->A[prec][0].p) {
+    const size_t half = clover_bytes(prec, geo.vh) / 2;
+    cf->A[prec][0].alloc(half);
+    cf->Ainv[prec][1].alloc(half);
+    launch_clover_convert(prec, cf->a(F64, 0), cf->A[prec][0].p, geo.vh, stream());
+    launch_clover_convert(prec, cf->ainv(F64, 1), cf->Ainv[prec][1].p, geo.vh, stream());
     QHIP(hipGetLastError());
   }
   return cf;
@@ -571,7 +579,7 @@ int qcuCloverExport(qcuGauge h, const qcuOperator* op, void* A, void* Ainv, int*
     ensure_init();
     QREQ(h && op, "qcuCloverExport: NULL argument");
     GaugeField& g = *reinterpret_cast<GaugeField*>(h);
-    CloverField* cf = g.clover(op->kappa * op->csw, F64);
+    CloverField* cf = g.clover(op->kappa * op->csw, F64, true);
     size_t half = (size_t)g.geo.vh * 72 * 16;
     for (int p = 0; p < 2; ++p) {
       if (A) launch_clover_unpack(cf->a(F64, p), static_cast<char*>(A) + p * half, g.geo.vh, stream());

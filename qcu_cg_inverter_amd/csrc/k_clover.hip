@@ -242,13 +242,15 @@ __global__ void clover_assemble_kernel(const double2* G, double2* A, double2* Ai
   for (int k = 0; k < 36; ++k) m[k] = ck * m[k];
 #pragma unroll
   for (int i = 0; i < 6; ++i) m[i * 7].re += 1.0;
-  zd inv[36];
-  if (!chol_inv6(m, inv)) {
-    gj_inv6(m, inv);
-    atomicAdd(nbad, 1);
+  if (A) pack_block(m, A, vh, site, b);
+  if (Ainv) {  // the even-odd solve needs the inverse on one parity only
+    zd inv[36];
+    if (!chol_inv6(m, inv)) {
+      gj_inv6(m, inv);
+      atomicAdd(nbad, 1);
+    }
+    pack_block(inv, Ainv, vh, site, b);
   }
-  pack_block(m, A, vh, site, b);
-  pack_block(inv, Ainv, vh, site, b);
 }
 
 // packed F64 -> F32 (18 float4 words per site) / F16 (9 uint4 words); one parity

@@ -49,16 +49,28 @@ struct DevBuf {
   char* c() const { return static_cast<char*>(p); }
 };
 
-// A = 1 + ck T and A^-1 of both parities, packed, per precision (F64 built first,
-// the others converted on demand)
+// A = 1 + ck T and A^-1, packed, per precision and parity, built on demand.  The
+// even-odd solve reads A on both parities and A^-1 on the odd one in FP64, and A_ee and
+// A^-1_oo in its inner precision; A^-1_ee (export, full-lattice compositions) is built
+// only when asked for.  Low precisions are converted from FP64.
 struct CloverField {
   double ck = 0.0;
-  int nbad = 0;
+  int nbad = 0;  // blocks inverted by the pivoted fallback, over the parities inverted so far
   long vh = 0;
-  DevBuf A[3], Ainv[3];
-  const void* a(int prec, int parity) const { return A[prec].c() + parity * clover_bytes(prec, vh) / 2; }
-  const void* ainv(int prec, int parity) const { return Ainv[prec].c() + parity * clover_bytes(prec, vh) / 2; }
+  DevBuf A[3][2], Ainv[3][2];
+  const void* a(int prec, int parity) const { return need(A[prec][parity]); }
+  const void* ainv(int prec, int parity) const { return need(Ainv[prec][parity]); }
+  static const void* need(const DevBuf& b) {
+    if (!b.p) throw Error(QCU_ERROR_STATE, "internal: clover part not built");
+    return b.p;
+  }
 };
+// device bytes of the clover parts an even-odd solve with inner precision prec uses
+inline size_t solver_clover_bytes(int prec, long vh) {
+  size_t s = 3 * (clover_bytes(F64, vh) / 2);
+  if (prec != F64) s += 2 * (clover_bytes(prec, vh) / 2);
+  return s;
+}
 
 // a gauge field resident in HBM: the interface copy plus packed variants made on
 // first use, and clover fields keyed by kappa * c_sw
@@ -69,7 +81,8 @@ struct GaugeField {
   DevBuf packed[3][2];  // [precision][recon == 12]
   std::map<double, std::unique_ptr<CloverField>> clovers;
   const void* get(int prec, int recon);
-  CloverField* clover(double ck, int prec);
+  // the solve's parts in F64 and prec; full_inverse adds A^-1_ee (FP64)
+  CloverField* clover(double ck, int prec, bool full_inverse = false);
   size_t bytes() const;
 };
 

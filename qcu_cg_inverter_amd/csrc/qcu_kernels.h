@@ -71,8 +71,8 @@ void launch_cg_xflush(int prec, int xprec, Vec x, Vec p, double alpha, long vh, 
 // --------------------------------------------------------------- clover -----
 // G[pl] = Q - Q^dag for the 6 planes at every site of `parity`: double2 [6][9][Vh]
 void launch_clover_leaves(const void* gauge_f64_r18, void* G, const Geom& g, int parity, hipStream_t st);
-// A = 1 + ck T and A^-1 for one parity, packed F64; nbad counts blocks that needed
-// the pivoted fallback instead of Cholesky
+// A = 1 + ck T and A^-1 for one parity, packed F64 (either output may be null);
+// nbad counts blocks that needed the pivoted fallback instead of Cholesky
 void launch_clover_assemble(const void* G, void* A, void* Ainv, long vh, double ck, int* nbad, hipStream_t st);
 // packed F64 -> packed F32 / F16 (one parity)
 void launch_clover_convert(int prec, const void* src, void* dst, long vh, hipStream_t st);

@@ -82,9 +82,7 @@ struct Server {
 
 size_t Server::clover_cost(GaugeField* g, double ck) {
   if (clover_refs.count({g, ck})) return 0;
-  size_t per = clover_bytes(F64, g->geo.vh) * 2;
-  if (cfg.precision != F64) per += clover_bytes(cfg.precision, g->geo.vh) * 2;
-  return per;
+  return solver_clover_bytes(cfg.precision, g->geo.vh);
 }
 
 void Server::release_clover(GaugeField* g, double ck) {
@@ -92,8 +90,7 @@ void Server::release_clover(GaugeField* g, double ck) {
   if (it == clover_refs.end()) return;
   if (--it->second > 0) return;
   clover_refs.erase(it);
-  size_t per = clover_bytes(F64, g->geo.vh) * 2;
-  if (cfg.precision != F64) per += clover_bytes(cfg.precision, g->geo.vh) * 2;
+  const size_t per = solver_clover_bytes(cfg.precision, g->geo.vh);
   auto c = g->clovers.find(ck);
   if (c != g->clovers.end()) {
     QHIP(hipStreamSynchronize(ctx().stream));

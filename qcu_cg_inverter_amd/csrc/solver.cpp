@@ -50,7 +50,7 @@ static OpView make_view(GaugeField& g, const qcuOperator& op, int prec) {
   o.op = op;
   o.clover = op.csw != 0.0;
   o.cf = nullptr;
-  if (o.clover) o.cf = g.clover(op.kappa * op.csw, prec);
+  if (o.clover) o.cf = g.clover(op.kappa * op.csw, prec, true);
   g.get(prec, op.recon);
   g.get(F64, op.recon);
   return o;
