@@ -126,3 +126,33 @@ def test_server_bad_request_fails_alone(q):
         so = s.poll(other)
         assert so.done and so.converged and so.true_residual <= 1e-10
     s.destroy()
+
+
+def test_server_stalled_request_not_starved(q):
+    """A request that needs memory held by a busy batch of another gauge gets in while
+    a backlog for that batch is still queued: after a short grace nothing behind it is
+    admitted, the busy batch drains and gives its memory back.  (Without the grace the
+    backlog would keep the batch busy and the request would wait for all of it.)"""
+    L = (8, 8, 8, 8)
+    g1 = q.Gauge.random(L, seed=51, eps=0.2)
+    g2 = q.Gauge.random(L, seed=52, eps=0.2)
+    # the budget of exactly one batch (measured with a server that makes one)
+    probe = q.Server(precision=1, max_batch=12, chunk=2)
+    probe.wait(probe.submit(g1, 0.12, 0.0, 1e-10, 1, seed=1))
+    one_batch = probe.usage()["bytes"]
+    probe.destroy()
+    assert one_batch > 0
+    s = q.Server(precision=1, max_batch=12, chunk=2, hbm_budget=one_batch + 4096)
+    first = s.submit(g1, 0.12, 0.0, 1e-12, 12, seed=100)
+    stalled = s.submit(g2, 0.12, 0.0, 1e-10, 1, seed=200)  # needs a second batch: no memory
+    backlog = [s.submit(g1, 0.12, 0.0, 1e-12, 12, seed=300 + k) for k in range(3000)]
+    st = s.wait(stalled)
+    pending = sum(1 for i in backlog if not s.poll(i).done)
+    assert st.converged and st.true_residual <= 1e-10
+    assert pending >= 1000, f"the stalled request waited for the backlog ({pending} of 3000 left)"
+    s.drain()
+    assert s.poll(first).converged and all(s.poll(i).converged for i in backlog)
+    assert s.usage()["peak_bytes"] <= one_batch + 4096
+    s.destroy()
+    g1.destroy()
+    g2.destroy()
