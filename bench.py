@@ -254,6 +254,12 @@ def config4_section(q):
                                   "reliable_updates": [s.reliable_updates for s in stsm],
                                   "max_true_residual": max(s.true_residual for s in stsm)}}
     out["speedup_solves_per_s"] = round(t64 / tm, 3)
+    # the same plain FP64 solves with graph-replayed iterations, for comparison
+    sp64g = q.default_solve_params(tol=1e-10, precision=0, graph=1)
+    q.solve_batch_device(g, op, xs[:1], srcs[:1], sp64g)  # warm (graph capture)
+    _, t64g = timed(lambda: [q.solve_batch_device(g, op, [xs[i]], [srcs[i]], sp64g)[0] for i in range(12)])
+    out["plain_fp64_sequential_graphs"] = {"seconds": round(t64g, 3), "solves_per_s": round(12 / t64g, 3),
+                                           "speedup_batched_mixed_fp32": round(t64g / tm, 3)}
     # beyond the configuration (which names FP32): the same batch with the FP16 +
     # per-site-norm inner solve
     q.tune_hop(g, 2, 12, 12)
@@ -264,8 +270,8 @@ def config4_section(q):
                                        "iterations": [s.iterations for s in stsh],
                                        "max_true_residual": max(s.true_residual for s in stsh),
                                        "speedup_solves_per_s": round(t64 / th, 3)}
-    log(f"config4: plain fp64 12 sequential {t64:.2f} s, batched mixed {tm:.2f} s, x{t64/tm:.2f}; "
-        f"fp16 inner {th:.2f} s, x{t64/th:.2f}")
+    log(f"config4: plain fp64 12 sequential {t64:.2f} s ({t64g:.2f} s with graphs), batched mixed {tm:.2f} s, "
+        f"x{t64/tm:.2f} (x{t64g/tm:.2f}); fp16 inner {th:.2f} s, x{t64/th:.2f}")
     g.destroy()
     return out
 
