@@ -1,7 +1,7 @@
-"""Parity at the largest configuration's volume (32^3x64, 2^20 sites per parity):
-the FP64 hop against the oracle, the 12-right-hand-side launch against 12 single
-launches bit for bit, the s = +1 / s = -1 adjointness of the hops, and the true
-residual of a mixed-precision solve recomputed by the oracle."""
+"""Parity at the largest configurations' volumes (32^3x64, 2^20 sites per parity, and
+24^3x48): the FP64 hop against the oracle, the 12-right-hand-side launch against 12
+single launches bit for bit, the s = +1 / s = -1 adjointness of the hops, and the
+true residuals of mixed-precision solves recomputed by the oracle."""
 import numpy as np
 import pytest
 
@@ -95,3 +95,20 @@ def test_config4_batch_true_residual_by_oracle(q, gauge):
         res = np.linalg.norm(b - orc.full_op(x, gref, L, kappa, A=A, s=1, tbc=-1.0, recon=12)) / np.linalg.norm(b)
         assert res <= 1e-10, (r, res)
         assert abs(res - sts[r].true_residual) <= 1e-3 * sts[r].true_residual
+
+
+def test_config3_fp16_solve_true_residual_by_oracle(q):
+    """24^3x48 clover (c_sw 1, kappa 0.122, eps 0.25), one Gaussian source, FP16 +
+    per-site-norm inner loop with FP64 reliable updates; the oracle recomputes the
+    residual with its own clover field."""
+    L3 = (24, 24, 24, 48)
+    g = q.Gauge.random(L3, seed=3, eps=0.25)
+    gref = g.to_numpy()
+    b = orc.source_gauss(L3, 2)
+    x, st = q.solve(g, q.Operator(0.122, 1.0, -1, 12), b, q.default_solve_params(tol=1e-10, precision=2))
+    assert st.converged and st.true_residual <= 1e-10
+    A = orc.clover_fields(gref, L3, 0.122, 1.0, recon=12)[0]
+    res = np.linalg.norm(b - orc.full_op(x, gref, L3, 0.122, A=A, s=1, tbc=-1.0, recon=12)) / np.linalg.norm(b)
+    assert res <= 1e-10
+    assert abs(res - st.true_residual) <= 1e-3 * st.true_residual
+    g.destroy()
