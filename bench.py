@@ -533,8 +533,15 @@ def main():
                 for recon in (18, 12):
                     ms = q.bench_hop(gl, precision=PREC[prec], recon=recon, nrhs=1, iters=100)
                     gbs = BYTES_PER_SITE[(prec, recon)] * vhl / (ms * 1e-3) / 1e9
-                    large[key][f"{prec}_r{recon}"] = {"us": round(ms * 1e3, 2), "GBps": round(gbs, 1),
-                                                       "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1)}
+                    row = {"us": round(ms * 1e3, 2), "GBps": round(gbs, 1),
+                           "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1)}
+                    # real traffic: PMC-measured bytes per launch (profiles/hop_f64_32x64.md)
+                    tb = measured_traffic(f"hop_{prec}_r{recon}_{key}")
+                    if tb:
+                        real = tb / (ms * 1e-3) / 1e9
+                        row.update({"traffic_bytes": tb, "real_GBps": round(real, 1),
+                                    "real_pct_hbm_peak": round(100 * real / HBM_PEAK_GBS, 1)})
+                    large[key][f"{prec}_r{recon}"] = row
                     log(f"hop {prec} r{recon} {key}: {ms*1e3:.1f} us, {gbs:.0f} GB/s")
             gl.destroy()
         out["dslash_large"] = large
