@@ -179,8 +179,7 @@ int qcuBenchCopy(size_t bytes, int iters, double* ms);
 typedef struct qcuServer_st* qcuServer;
 
 typedef struct {
-  size_t hbm_budget;  /* device bytes for batches and clover fields (0: free memory at
-                         creation less a reserve of max(16 GiB, 1/16 of it)) */
+  size_t hbm_budget;  /* device bytes for batches and clover fields (0: 3/4 of free memory) */
   int max_batch;      /* right-hand-side slots per batch, 1..16 */
   int precision;      /* inner precision of every solve (QCU_PREC_*) */
   int chunk;          /* CG iterations per scheduling round */

@@ -14,7 +14,6 @@
 // work needs their memory.  A right-hand side enters at a chunk boundary and every
 // decision about it uses only its own numbers, so its result is the same whichever
 // requests share its batch.
-#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -366,10 +365,7 @@ int qcuServerCreate(qcuServer* out, const qcuServerConfig* cfg) {
     if (s->budget == 0) {
       size_t fr = 0, tot = 0;
       QHIP(hipMemGetInfo(&fr, &tot));
-      // all free memory but a reserve for what the library allocates outside the
-      // budget (clover build scratch, graphs, tuning fields)
-      const size_t reserve = std::max<size_t>((size_t)16 << 30, fr / 16);
-      s->budget = fr > 2 * reserve ? fr - reserve : fr / 2;
+      s->budget = fr / 4 * 3;
     }
     Server* raw = s.get();
     s->worker = std::thread([raw] { raw->run(); });
