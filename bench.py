@@ -492,6 +492,16 @@ def main():
                        "kernel": "hop_kernel<F64,18,EPI_PLAIN>", "avg_launch_us": round(ms_event * 1e3, 3),
                        "algorithmic_bytes_per_launch": bps}
     log(f"hop f64 r18 16^3x32: {ms_event*1e3:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
+    # the same hop as the median of 100 windows of 20 back-to-back launches, each
+    # window timed by its own pair of HIP events
+    win = sorted(q.bench_hop(g, precision=0, recon=18, nrhs=1, iters=20) for _ in range(100))
+    med = 0.5 * (win[49] + win[50])
+    out["dslash_16x32_median"] = {"windows": 100, "launches_per_window": 20,
+                                  "median_us": round(med * 1e3, 3), "min_us": round(win[0] * 1e3, 3),
+                                  "max_us": round(win[-1] * 1e3, 3),
+                                  "GBps": round(bps / (med * 1e-3) / 1e9, 1),
+                                  "pct_hbm_peak": round(100 * bps / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+    log(f"hop f64 r18 16^3x32 median of 100 windows: {med*1e3:.2f} us/launch")
 
     # ---- copy bandwidth -----------------------------------------------------------
     nb = 1 << 30
