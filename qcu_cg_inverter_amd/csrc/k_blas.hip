@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kSumThreads) sum_partials_kernel(const double*
 }
 
 template <int P>
-__global__ void axpby_kernel(double a, Vec x, double b, Vec y, long vh) {
+__global__ void axpby_kernel(double a, Vec x, double b, Vec y, long vh, int nts) {
   using T = typename Real<P>::T;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int r = blockIdx.y;
@@ -110,7 +110,8 @@ __global__ void axpby_kernel(double a, Vec x, double b, Vec y, long vh) {
   SpinorIO<P>::load(yp, vh, i, yv);
 #pragma unroll
   for (int c = 0; c < 12; ++c) yv[c] = T(a) * xv[c] + T(b) * yv[c];
-  SpinorIO<P>::store(yp, vh, i, yv);
+  if (nts) SpinorIO<P>::template store<1>(yp, vh, i, yv);
+  else SpinorIO<P>::template store<0>(yp, vh, i, yv);
 }
 
 template <int P>
@@ -178,7 +179,7 @@ __global__ void __launch_bounds__(kSumThreads) cg_ss_kernel(const double* part, 
 // saves one read of p per iteration against doing it in the s update; the host
 // applies a pending update itself (cg_xflush) before it reads x.
 template <int P, int PX>
-__global__ void cg_xpay_kernel(Vec r, Vec p, Vec x, const CgScal* sc, long vh) {
+__global__ void cg_xpay_kernel(Vec r, Vec p, Vec x, const CgScal* sc, long vh, int nts) {
   using T = typename Real<P>::T;
   using TX = typename Real<PX>::T;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,14 +201,16 @@ __global__ void cg_xpay_kernel(Vec r, Vec p, Vec x, const CgScal* sc, long vh) {
         SpinorIO<F64>::load_half<1>(xp, vh, i, h, xv);
 #pragma unroll
         for (int c = 0; c < 6; ++c) xv[c] = {xv[c].re + alpha * pv[c].re, xv[c].im + alpha * pv[c].im};
-        SpinorIO<F64>::store_half(xp, vh, i, h, xv);
+        if (nts) SpinorIO<F64>::store_half<1>(xp, vh, i, h, xv);
+        else SpinorIO<F64>::store_half<0>(xp, vh, i, h, xv);
       }
       SpinorIO<F64>::load_half<1>(rp, vh, i, h, rv);
       if (beta != T(0)) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) rv[c] = rv[c] + (double)beta * pv[c];
       }
-      SpinorIO<F64>::store_half(pp, vh, i, h, rv);
+      if (nts) SpinorIO<F64>::store_half<1>(pp, vh, i, h, rv);
+      else SpinorIO<F64>::store_half<0>(pp, vh, i, h, rv);
     }
     return;
   }
@@ -219,23 +222,26 @@ __global__ void cg_xpay_kernel(Vec r, Vec p, Vec x, const CgScal* sc, long vh) {
     SpinorIO<PX>::load(xp, vh, i, xv);
 #pragma unroll
     for (int c = 0; c < 12; ++c) xv[c] = {xv[c].re + ax * TX(pv[c].re), xv[c].im + ax * TX(pv[c].im)};
-    SpinorIO<PX>::store(xp, vh, i, xv);
+    if (nts) SpinorIO<PX>::template store<1>(xp, vh, i, xv);
+    else SpinorIO<PX>::template store<0>(xp, vh, i, xv);
   }
   cx<T> rv[12];
   SpinorIO<P>::load(rp, vh, i, rv);
   if (beta == T(0)) {
     // restart (first iteration, after a restart, or converged): p = r
-    SpinorIO<P>::store(pp, vh, i, rv);
-    return;
-  }
 #pragma unroll
-  for (int c = 0; c < 12; ++c) pv[c] = rv[c] + beta * pv[c];
-  SpinorIO<P>::store(pp, vh, i, pv);
+    for (int c = 0; c < 12; ++c) pv[c] = rv[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 12; ++c) pv[c] = rv[c] + beta * pv[c];
+  }
+  if (nts) SpinorIO<P>::template store<1>(pp, vh, i, pv);
+  else SpinorIO<P>::template store<0>(pp, vh, i, pv);
 }
 
 // s -= alpha t, with the 16-site partials of |s|^2
 template <int P>
-__global__ void cg_update_kernel(Vec t, Vec s, const CgScal* sc, double* part, long vh) {
+__global__ void cg_update_kernel(Vec t, Vec s, const CgScal* sc, double* part, long vh, int nts) {
   using T = typename Real<P>::T;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y;
@@ -257,7 +263,8 @@ __global__ void cg_update_kernel(Vec t, Vec s, const CgScal* sc, double* part, l
           sv[c] = sv[c] - (double)a * tv[c];
           n += T(norm2(sv[c]));
         }
-        SpinorIO<F64>::store_half(sp, vh, i, h, sv);
+        if (nts) SpinorIO<F64>::store_half<1>(sp, vh, i, h, sv);
+        else SpinorIO<F64>::store_half<0>(sp, vh, i, h, sv);
       }
     } else {
       cx<T> tv[12], sv[12];
@@ -268,7 +275,8 @@ __global__ void cg_update_kernel(Vec t, Vec s, const CgScal* sc, double* part, l
         sv[c] = sv[c] - a * tv[c];
         n += norm2(sv[c]);
       }
-      SpinorIO<P>::store(sp, vh, i, sv);
+      if (nts) SpinorIO<P>::template store<1>(sp, vh, i, sv);
+      else SpinorIO<P>::template store<0>(sp, vh, i, sv);
     }
     acc = (double)n;
   }
@@ -324,7 +332,7 @@ void launch_sum_partials(const double* part, long nw, int nrhs, double* out, hip
   hipLaunchKernelGGL(sum_partials_kernel, dim3(nrhs), dim3(kSumThreads), 0, st, part, nw, out);
 }
 void launch_axpby(int prec, double a, Vec x, double b, Vec y, long vh, int nrhs, hipStream_t st) {
-  QCU_PSW(prec, axpby_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, a, x, b, y, vh);
+  QCU_PSW(prec, axpby_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, a, x, b, y, vh, nt_store(prec, vh, nrhs));
 }
 void launch_clover_apply(int prec, const void* clov, Vec in, Vec out, long vh, int nrhs, hipStream_t st) {
   QCU_PSW(prec, clover_apply_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, clov, in, out, vh);
@@ -340,11 +348,12 @@ void launch_cg_ss(const double* part, long nw, CgScal* sc, int nrhs, hipStream_t
 }
 void launch_cg_xpay(int prec, int xprec, Vec r, Vec p, Vec x, const CgScal* sc, long vh, int nrhs, hipStream_t st) {
   dim3 g(nblk(vh, 256), nrhs), b(256);
-  if (prec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F64, F64>), g, b, 0, st, r, p, x, sc, vh);
-  else if (prec == F32 && xprec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F32, F64>), g, b, 0, st, r, p, x, sc, vh);
-  else if (prec == F32) hipLaunchKernelGGL((cg_xpay_kernel<F32, F32>), g, b, 0, st, r, p, x, sc, vh);
-  else if (xprec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F16, F64>), g, b, 0, st, r, p, x, sc, vh);
-  else hipLaunchKernelGGL((cg_xpay_kernel<F16, F32>), g, b, 0, st, r, p, x, sc, vh);
+  const int n = nt_store(prec, vh, nrhs);  // p decides for x as well
+  if (prec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F64, F64>), g, b, 0, st, r, p, x, sc, vh, n);
+  else if (prec == F32 && xprec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F32, F64>), g, b, 0, st, r, p, x, sc, vh, n);
+  else if (prec == F32) hipLaunchKernelGGL((cg_xpay_kernel<F32, F32>), g, b, 0, st, r, p, x, sc, vh, n);
+  else if (xprec == F64) hipLaunchKernelGGL((cg_xpay_kernel<F16, F64>), g, b, 0, st, r, p, x, sc, vh, n);
+  else hipLaunchKernelGGL((cg_xpay_kernel<F16, F32>), g, b, 0, st, r, p, x, sc, vh, n);
 }
 void launch_cg_xflush(int prec, int xprec, Vec x, Vec p, double alpha, long vh, hipStream_t st) {
   dim3 g(nblk(vh, 256)), b(256);
@@ -355,7 +364,8 @@ void launch_cg_xflush(int prec, int xprec, Vec x, Vec p, double alpha, long vh, 
   else hipLaunchKernelGGL((cg_xflush_kernel<F16, F32>), g, b, 0, st, x, p, alpha, vh);
 }
 void launch_cg_update(int prec, Vec t, Vec s, const CgScal* sc, double* part, long vh, int nrhs, hipStream_t st) {
-  QCU_PSW(prec, cg_update_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, t, s, sc, part, vh);
+  QCU_PSW(prec, cg_update_kernel, dim3(nblk(vh, 256), nrhs), dim3(256), 0, st, t, s, sc, part, vh,
+          nt_store(prec, vh, nrhs));
 }
 
 }  // namespace qcu

@@ -92,6 +92,16 @@ inline size_t spinor_bytes(int prec, long vh) {
   const size_t n = (size_t)padded_sites(vh);
   return prec == F64 ? n * 192 : prec == F32 ? n * 96 : n * 52;
 }
+// 1: a launch writing `nrhs` fields of `prec` stores them nontemporally, and a hop
+// reads its epilogue's x that way too.  Streamed fields that overflow about half the
+// 256 MB Infinity Cache would only evict what the next kernel reads; smaller ones are
+// served from it to that kernel.  Measured on 32^3x64: nontemporal stores make the
+// FP64 hop 5-7 % and the 12-RHS FP32 hop 4 % faster, and the 1-RHS FP32 hop (100 MB
+// written) 6 % slower; on 16^3x32 the FP64 hop (13 MB) 7 % slower.
+constexpr size_t kNtStoreBytes = (size_t)128 << 20;
+inline int nt_store(int prec, long vh, int nrhs) {
+  return (size_t)nrhs * spinor_bytes(prec, vh) > kNtStoreBytes ? 1 : 0;
+}
 inline int link_entries(int recon) { return recon == 12 ? 6 : 9; }
 inline size_t gauge_bytes(int prec, int recon, long vh) {
   size_t cplx = (size_t)8 * link_entries(recon) * padded_sites(vh);  // 4 mu x 2 parities, site blocks
@@ -137,6 +147,7 @@ struct HopArgs {
   int cached;                            // 1: ordinary loads of links / clover / x, 0: streaming
   long nsb;                              // site blocks of the lattice (set by the launcher)
   int clov_shared;                       // every right-hand side uses clov[0]
+  int nt_store;                          // nontemporal stores of out and loads of x (set by the launcher)
   Walk walk;                             // site traversal order
 };
 

@@ -82,6 +82,23 @@ __device__ __forceinline__ uint2 ld_nt(const uint2* p) {
   return uint2{v.x, v.y};
 }
 __device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return __builtin_nontemporal_load(p); }
+// nontemporal (streaming) vector stores
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  __builtin_nontemporal_store(v2d_t{v.x, v.y}, reinterpret_cast<v2d_t*>(p));
+}
+__device__ __forceinline__ void st_nt(float4* p, float4 v) {
+  __builtin_nontemporal_store(v4f_t{v.x, v.y, v.z, v.w}, reinterpret_cast<v4f_t*>(p));
+}
+__device__ __forceinline__ void st_nt(uint4* p, uint4 v) {
+  __builtin_nontemporal_store(v4u_t{v.x, v.y, v.z, v.w}, reinterpret_cast<v4u_t*>(p));
+}
+__device__ __forceinline__ void st_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+// NTS = 1: nontemporal store (nt_store() in qcu_common.h says when); NTS = 0: ordinary
+template <int NTS, typename V>
+__device__ __forceinline__ void stp(V* p, V v) {
+  if (NTS) st_nt(p, v);
+  else *p = v;
+}
 // NT = 1: nontemporal (streaming) load, which keeps data read once out of the caches'
 // way; NT = 0: ordinary load, which lets a field that fits the 256 MB Infinity Cache
 // be served from there on the next launch.  Which is faster depends on the lattice, so
@@ -148,15 +165,17 @@ struct SpinorIO<F64> {
       v[c] = {q.x, q.y};
     }
   }
+  template <int NTS = 0>
   static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
     double2* b = site_block<12>(static_cast<double2*>(base), i);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) b[c * kSiteBlock] = double2{v[c].re, v[c].im};
+    for (int c = 0; c < 12; ++c) stp<NTS>(b + c * kSiteBlock, double2{v[c].re, v[c].im});
   }
+  template <int NTS = 0>
   static __device__ __forceinline__ void store_half(void* base, long vh, long i, int h, const cx<T> v[6]) {
     double2* b = site_block<12>(static_cast<double2*>(base), i) + 6 * h * kSiteBlock;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) b[c * kSiteBlock] = double2{v[c].re, v[c].im};
+    for (int c = 0; c < 6; ++c) stp<NTS>(b + c * kSiteBlock, double2{v[c].re, v[c].im});
   }
 };
 
@@ -185,21 +204,22 @@ struct SpinorIO<F32> {
       v[2 * k + 1] = {q.z, q.w};
     }
   }
-  template <int NT = 1>  // FP32 spinors are always loaded cached in the hop
+  template <int NT = 0>
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const float4* b = at(base, i) + 3 * h * kSiteBlock;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      float4 q = b[k * kSiteBlock];
+      float4 q = ldp<NT>(b + k * kSiteBlock);
       v[2 * k] = {q.x, q.y};
       v[2 * k + 1] = {q.z, q.w};
     }
   }
+  template <int NTS = 0>
   static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
     float4* b = site_block<6>(static_cast<float4*>(base), i);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      b[k * kSiteBlock] = float4{v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im};
+      stp<NTS>(b + k * kSiteBlock, float4{v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im});
   }
 };
 
@@ -255,19 +275,21 @@ struct SpinorIO<F16> {
     }
   }
   // components 0..5 are word 0 and the first half of word 1; 6..11 the rest
-  template <int NT = 1>  // FP16 spinors are always loaded cached in the hop
+  template <int NT = 0>
   static __device__ __forceinline__ void load_half(const void* base, long vh, long i, int h, cx<T> v[6]) {
     const uint4* b = at(base, i) + h * kSiteBlock;
-    float nrm = norms(base, vh)[i];
+    const unsigned* np = reinterpret_cast<const unsigned*>(norms(base, vh) + i);
+    float nrm = __uint_as_float(ldp<NT>(np));
     float f[16];
-    unpack8(b[0], f);
-    unpack8(b[kSiteBlock], f + 8);
+    unpack8(ldp<NT>(b), f);
+    unpack8(ldp<NT>(b + kSiteBlock), f + 8);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const int o = h ? 2 * c + 4 : 2 * c;
       v[c] = {f[o] * nrm, f[o + 1] * nrm};
     }
   }
+  template <int NTS = 0>
   static __device__ __forceinline__ void store(void* base, long vh, long i, const cx<T> v[12]) {
     float m = 0.f;
 #pragma unroll
@@ -282,9 +304,9 @@ struct SpinorIO<F16> {
         f[2 * j] = v[4 * k + j].re * inv;
         f[2 * j + 1] = v[4 * k + j].im * inv;
       }
-      b[k * kSiteBlock] = pack8(f);
+      stp<NTS>(b + k * kSiteBlock, pack8(f));
     }
-    reinterpret_cast<float*>(static_cast<uint4*>(base) + 3 * padded_sites(vh))[i] = m;
+    stp<NTS>(reinterpret_cast<float*>(static_cast<uint4*>(base) + 3 * padded_sites(vh)) + i, m);
   }
 };
 

@@ -227,6 +227,25 @@ def test_batch_equals_alone(q, prec):
         assert sts[i].true_residual <= 1e-10
 
 
+def test_zero_source(q):
+    """b = 0 gives x = 0 after no iterations with true residual 0, alone and in a batch
+    beside nonzero sources, whose bits it leaves unchanged."""
+    L = (8, 8, 4, 8)
+    g = q.Gauge.random(L, seed=8, eps=0.25)
+    op = q.Operator(0.122, 1.0, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=1)
+    b = orc.source_gauss(L, 100)
+    z = np.zeros_like(b)
+    x, st = q.solve(g, op, z, sp)
+    assert st.converged and st.iterations == 0 and st.true_residual == 0.0
+    assert not np.any(x)
+    xs, sts = q.solve_batch(g, op, [b, z, b], sp)
+    assert not np.any(xs[1]) and sts[1].iterations == 0 and sts[1].converged
+    xa, sa = q.solve(g, op, b, sp)
+    assert np.array_equal(xs[0], xa) and np.array_equal(xs[2], xa)
+    assert sts[0].iterations == sa.iterations == sts[2].iterations
+
+
 def test_graph_equals_eager(q):
     L = (8, 8, 8, 8)
     g = q.Gauge.random(L, seed=6, eps=0.2)
