@@ -347,7 +347,32 @@ def cpu_baseline(seconds_target=15.0):
         orc.hop(v3[1 - (n3 & 1)], g3, L3, n3 & 1, 1, -1.0, 18)
         n3 += 1
     el3 = time.perf_counter() - t0
-    del g3, v3
+    # configuration 3's FP64 clover solve on the same inputs (generators seeded as on
+    # the GPU), as a bounded sample: the clover build, then the cost of one CG
+    # iteration from the difference of a 5- and a 10-iteration run
+    g3 = orc.gauge_random(L3, 5, 0.25)
+    b3 = orc.source_gauss(L3, 6)
+    t0 = time.perf_counter()
+    A3, Ainv3 = orc.clover_fields(g3, L3, 0.122, 1.0, recon=12)
+    tcl3 = time.perf_counter() - t0
+    tit = []
+    for m in (5, 10):
+        t0 = time.perf_counter()
+        orc.cg_eo(b3, g3, L3, 0.122, A=A3, Ainv=Ainv3, tbc=-1.0, recon=12, tol=1e-10, max_iter=m)
+        tit.append(time.perf_counter() - t0)
+    it3 = max(tit[1] - tit[0], 1e-9) / 5
+    del g3, v3, b3, A3, Ainv3
+    # the 32^3x64 hop (configuration 4's lattice) on all threads
+    L4 = (32, 32, 32, 64)
+    g4 = orc.gauge_random(L4, 3, 0.25)
+    v4 = orc.source_gauss(L4, 2)
+    orc.hop(v4[1], g4, L4, 0, 1, -1.0, 18)
+    n4, t0 = 0, time.perf_counter()
+    while n4 < 2 or time.perf_counter() - t0 < 3.0:
+        orc.hop(v4[1 - (n4 & 1)], g4, L4, n4 & 1, 1, -1.0, 18)
+        n4 += 1
+    el4 = time.perf_counter() - t0
+    del g4, v4
     # the CPU-runnable configuration: 8^4 Wilson, kappa 0.12, eps 0.1, antiperiodic t,
     # one Gaussian source, FP64 even-odd CG to 1e-10
     L1 = (8, 8, 8, 8)
@@ -364,6 +389,14 @@ def cpu_baseline(seconds_target=15.0):
                          "ms_per_apply": round(1e3 * el1 / n1, 3), "applications": n1},
             "hop_24x24x24x48": {"GBps": round(BYTES_PER_SITE[("f64", 18)] * orc.half_volume(L3) * n3 / el3 / 1e9, 3),
                                 "ms_per_apply": round(1e3 * el3 / n3, 3), "applications": n3, "cores": threads},
+            "hop_32x32x32x64": {"GBps": round(BYTES_PER_SITE[("f64", 18)] * orc.half_volume(L4) * n4 / el4 / 1e9, 3),
+                                "ms_per_apply": round(1e3 * el4 / n4, 3), "applications": n4, "cores": threads},
+            "cg_config3_24x24x24x48_clover_sample": {
+                "clover_build_seconds": round(tcl3, 3), "ms_per_iteration": round(1e3 * it3, 2),
+                "projected_seconds_100_iterations": round(tcl3 + tit[1] + 90 * it3, 2), "cores": threads,
+                "sample": "clover build, then FP64 CG runs of 5 and 10 iterations (per-iteration cost from their "
+                          "difference; the projection is the build, the 10-iteration run and 90 more iterations); "
+                          "the GPU's plain FP64 solve of the same system takes 100 iterations"},
             "host_cpu": platform.processor() or platform.machine(),
             "cg_config1_8x8x8x8": {"seconds": round(tcg, 4), "iterations": st.iterations,
                                    "true_residual": st.true_residual},
