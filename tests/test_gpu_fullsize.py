@@ -40,8 +40,11 @@ def test_hop_f64_32x64_matches_oracle(q, gauge):
         assert rel(out, ref) < 1e-14, (parity, dag, recon)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_hop_12rhs_32x64_equals_single_launches(q, gauge, prec):
+    """12 right-hand sides in one launch give the bits of 12 single launches; at this
+    volume the batch's fields are stored nontemporally and a single FP32 / FP16 field
+    is not, so this also checks that the store policy leaves the values alone."""
     vh = q.half_volume(L)
     rng = np.random.default_rng(5)
     vs = [(rng.standard_normal((vh, 4, 3)) + 1j * rng.standard_normal((vh, 4, 3))) for _ in range(12)]
