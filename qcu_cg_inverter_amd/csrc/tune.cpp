@@ -4,8 +4,9 @@
 // candidate shape -- block size, wave groups per block sharing sites, right-hand
 // sides per wave, group order, XCD-contiguous block order, walk tile -- on scratch
 // fields with hipEvents and the plain epilogue, keeps the fastest, and re-times the
-// eight fastest with each epilogue that reads x or a clover field (those add loads
-// that can favour another shape).  Results live in the context's table and, when
+// eight fastest, and the two fastest that take every right-hand side in one block per
+// site range, with each epilogue that reads x or a clover field (those add loads that
+// can favour another shape).  Results live in the context's table and, when
 // QCU_TUNE_CACHE names a file, are appended there and reloaded by later processes.
 #include <algorithm>
 #include <cstdio>
@@ -164,15 +165,39 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   std::string key = tune_key(prec, recon, vh, nrhs, EPI_PLAIN);
   ctx().tune[key] = best;
   append_cache(key, best, bms);
-  // ... and the fastest few again with each epilogue that reads x or the clover
+  // ... and the fastest few again with each epilogue that reads x or the clover, plus
+  // the two fastest shapes that serve every right-hand side from one block per site
+  // range (links and clover read once per launch), streaming and cached: the epilogue
+  // loads can favour them although they lose with the plain one
+  std::vector<Tune> epi_shapes;
+  auto same = [](const Tune& x, const Tune& y) {
+    return x.block == y.block && x.rb == y.rb && x.rpt == y.rpt && x.xcd == y.xcd && x.tz == y.tz &&
+           x.tt == y.tt && x.gm == y.gm && x.ty == y.ty && x.rpw == y.rpw && x.spt == y.spt && x.cached == y.cached;
+  };
+  auto add_shape = [&](const Tune& t) {
+    for (const Tune& u : epi_shapes)
+      if (same(t, u)) return;
+    epi_shapes.push_back(t);
+  };
+  for (size_t k = 0; k < top; ++k) add_shape(timed[k].second);
+  int one_group = 0;
+  for (size_t k = 0; k < timed.size() && one_group < 2; ++k) {
+    Tune t = timed[k].second;
+    if (t.rb * t.rpw * t.rpt < nrhs || t.spt != 1) continue;
+    t.cached = 0;
+    add_shape(t);
+    t.cached = 1;
+    add_shape(t);
+    ++one_group;
+  }
   for (int epi = EPI_XPAY; epi <= EPI_CLOV_XPAY; ++epi) {
     Tune be = best;
     double bt = 1e30;
-    for (size_t k = 0; k < top; ++k) {
-      double ms = time_shape(timed[k].second, epi);
+    for (const Tune& t : epi_shapes) {
+      double ms = time_shape(t, epi);
       if (ms < bt) {
         bt = ms;
-        be = timed[k].second;
+        be = t;
       }
     }
     std::string ke = tune_key(prec, recon, vh, nrhs, epi);
