@@ -98,6 +98,10 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   DevBuf* b = ctx().buf("tune_b", fb * nrhs);
   launch_source_gauss(src->p, vh * 12 * nrhs, 777, stream());
   launch_spinor_from_aos(prec, src->p, vh * 192, vec(a->p, (long)fb), vh, nrhs, stream());
+  // x of the xpay epilogues: a field of its own, as in a solve (not the hop's input,
+  // whose loads would hit the caches)
+  DevBuf* xb = ctx().buf("tune_x", fb * nrhs);
+  launch_spinor_from_aos(prec, src->p, vh * 192, vec(xb->p, (long)fb), vh, nrhs, stream());
   // a clover-sized scratch field for the clover epilogues (its values do not matter)
   DevBuf* cl = ctx().buf("tune_clov", clover_bytes(prec, vh) / 2);
   QHIP(hipMemsetAsync(cl->p, 0, clover_bytes(prec, vh) / 2, stream()));
@@ -106,7 +110,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   memset(&args, 0, sizeof args);
   args.in = a->p;
   args.out = b->p;
-  args.x = a->p;
+  args.x = xb->p;
   args.in_stride = args.out_stride = args.x_stride = (long)fb;
   args.gauge = gauge;
   for (int r = 0; r < nrhs; ++r) {
