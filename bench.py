@@ -276,6 +276,45 @@ def config4_section(q):
     return out
 
 
+def dropin_section(q, g):
+    """The reference's entry points as a caller sees them, on the 16^3x32 gauge of the
+    headline: host-synchronous dslashQcu (one parity, s = +1, periodic, kappa = 1) and
+    fullDslashQcu (A - D with kappa = c_sw = 1, even-first full spinors), each call
+    including the interface <-> internal layout conversions."""
+    import ctypes
+
+    import numpy as np
+
+    L = g.L
+    vh = q.half_volume(L)
+    gd = q.DeviceBuffer.from_numpy(g.to_numpy())
+    prm = q.param(L)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    q.lib().loadQcuGauge(gd.ptr, ctypes.byref(prm))
+    full = q.source_gaussian_device(L, seed=7)
+    out = q.DeviceBuffer(2 * vh * 192)
+    res = {}
+    for name, n in (("dslashQcu", 200), ("fullDslashQcu", 100)):
+        def call():
+            if name == "dslashQcu":
+                q.lib().dslashQcu(out.ptr, full.ptr + vh * 192, gd.ptr, ctypes.byref(prm), 0)
+            else:
+                q.lib().fullDslashQcu(out.ptr, full.ptr, gd.ptr, ctypes.byref(prm), 0)
+        for _ in range(5):
+            call()  # warm (clover build on the first fullDslashQcu)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            call()
+        us = 1e6 * (time.perf_counter() - t0) / n
+        res[name] = {"us_per_call": round(us, 2), "calls": n}
+        log(f"drop-in {name} 16^3x32: {us:.1f} us per call")
+    q.check(q.lib().qcuGetLastError())
+    res["note"] = ("host-synchronous calls through the drop-in symbols, including the layout conversions; "
+                   "the hop kernel alone is roofline.avg_launch_us")
+    del np
+    return res
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -540,6 +579,7 @@ def main():
     nb = 1 << 30
     ms = q.bench_copy(nb, iters=20)
     out["copy"] = {"bytes": nb, "GBps_read_plus_write": round(2 * nb / (ms * 1e-3) / 1e9, 1)}
+    out["dropin_16x32"] = dropin_section(q, g)
 
     # ---- CG: FP64 vs mixed on 16^3x32 Wilson ----------------------------------------
     b = q.source_gaussian(L_MAIN, seed=2)
