@@ -16,6 +16,10 @@ The same JSON line also carries:
   dslash_large  the same kernel on 24^3x48 and 32^3x64 (working set above the
                 256 MB Infinity Cache), FP64 / FP32 / FP16 and reconstruct-12
   copy          device-to-device copy bandwidth for scale
+  dslash_16x32_median  the headline hop as the median of 100 timed windows
+  dropin_16x32  host-synchronous dslashQcu / fullDslashQcu calls through the drop-in
+  config1_..., dslash_12rhs, config3_..., config4_..., config5_serving
+                the other configurations: solves, the 12-RHS hop, serving
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
