@@ -1,7 +1,7 @@
 """Parity at the largest configurations' volumes (32^3x64, 2^20 sites per parity, and
 24^3x48): the FP64 hop against the oracle, the 12-right-hand-side launch against 12
 single launches bit for bit, the s = +1 / s = -1 adjointness of the hops, and the
-true residuals of mixed-precision solves recomputed by the oracle."""
+true residuals of FP64 and mixed-precision solves recomputed by the oracle."""
 import numpy as np
 import pytest
 
@@ -75,6 +75,20 @@ def test_mixed_solve_32x64_true_residual_by_oracle(q, gauge):
     assert st.converged and st.true_residual <= 1e-10
     r = b - orc.full_op(x, gref, L, 0.122, s=1, tbc=-1.0, recon=12)
     res = np.linalg.norm(r) / np.linalg.norm(b)
+    assert res <= 1e-10
+    assert abs(res - st.true_residual) <= 1e-3 * st.true_residual
+
+
+def test_fp64_solve_32x64_true_residual_by_oracle(q, gauge):
+    """Plain FP64 clover solve on the full volume, whose vector kernels store their
+    201 MB fields nontemporally; the oracle recomputes the residual with its own
+    clover field."""
+    gref = gauge.to_numpy()
+    b = orc.source_gauss(L, 13)
+    x, st = q.solve(gauge, q.Operator(0.122, 1.0, -1, 12), b, q.default_solve_params(tol=1e-10, precision=0))
+    assert st.converged and st.true_residual <= 1e-10
+    A = orc.clover_fields(gref, L, 0.122, 1.0, recon=12)[0]
+    res = np.linalg.norm(b - orc.full_op(x, gref, L, 0.122, A=A, s=1, tbc=-1.0, recon=12)) / np.linalg.norm(b)
     assert res <= 1e-10
     assert abs(res - st.true_residual) <= 1e-3 * st.true_residual
 
