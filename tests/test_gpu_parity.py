@@ -156,6 +156,31 @@ def test_dropin_dslash_and_full(q):
         assert rel(res, ref) < FP64_TOL
 
 
+def test_dropin_dslash_16x32(q):
+    """The headline lattice through the drop-in symbols: dslashQcu on both parities and
+    fullDslashQcu with s = -1, against the oracle (periodic, kappa = c_sw = 1)."""
+    L = (16, 16, 16, 32)
+    gref = orc.gauge_random(L, 22, 0.2)
+    prm = q.param(L)
+    gd = dev(gref)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    q.lib().loadQcuGauge(gd.ptr, ctypes.byref(prm))
+    vh = orc.half_volume(L)
+    v = orc.source_gauss(L, 5)
+    for parity in (0, 1):
+        inp = dev(v[1 - parity])
+        out = q.DeviceBuffer(vh * 192)
+        q.lib().dslashQcu(out.ptr, inp.ptr, gd.ptr, ctypes.byref(prm), parity)
+        ref = orc.hop(v[1 - parity], gref, L, parity, s=1, tbc=1.0, recon=12)
+        assert rel(out.to_numpy((vh, 4, 3)), ref) < FP64_TOL
+    A = orc.clover_fields(gref, L, 1.0, 1.0, recon=12)[0]
+    inp = dev(v)
+    out = q.DeviceBuffer(2 * vh * 192)
+    q.lib().fullDslashQcu(out.ptr, inp.ptr, gd.ptr, ctypes.byref(prm), 1)
+    ref = orc.full_op(v, gref, L, 1.0, A=A, s=-1, tbc=1.0, recon=12)
+    assert rel(out.to_numpy((2, vh, 4, 3)), ref) < FP64_TOL
+
+
 def test_dropin_cg_inverter(q):
     L = (4, 4, 4, 8)
     gref = orc.gauge_random(L, 5, 0.25)
