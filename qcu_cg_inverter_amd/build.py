@@ -21,6 +21,7 @@ LIB = os.path.join(HERE, "libqcu.so")
 SOURCES = [
     *[f"k_hop_{p}_{r}.hip" for p in ("f64", "f32", "f16") for r in (1, 2, 3, 4)],
     "k_hop.hip",
+    "k_hop_iface.hip",
     "k_layout.hip",
     "k_blas.hip",
     "k_clover.hip",

@@ -294,20 +294,69 @@ using namespace qcu;
 // =================================================================== qcu.h ===
 static QcuParam g_param;
 
+// fingerprint of the caller's interface gauge (launch_fingerprint), on the host
+static uint64_t gauge_fingerprint(const void* aos, long vh) {
+  DevBuf* s = ctx().buf("fingerprint", sizeof(uint64_t) * (kFingerprintBlocks + 1));
+  launch_fingerprint(aos, (size_t)8 * vh * 9 * 16, static_cast<uint64_t*>(s->p), stream());
+  QHIP(hipGetLastError());
+  uint64_t h = 0;
+  QHIP(hipMemcpyAsync(&h, s->p, sizeof h, hipMemcpyDeviceToHost, stream()));
+  QHIP(hipStreamSynchronize(stream()));
+  return h;
+}
+
+// 1: the drop-in calls trust that a gauge pointer they have seen still holds the same
+// links (the copy is remade only by initGridSize / loadQcuGauge or a new pointer)
+static bool dropin_trust_pointer() {
+  const char* e = getenv("QCU_DROPIN_GAUGE_CACHE");
+  return e && strcmp(e, "0") != 0;
+}
+
+static void require_aligned(const void* p, const char* what) {
+  QREQ((reinterpret_cast<uintptr_t>(p) & 15) == 0, std::string(what) + " must be 16-byte aligned (complex128)");
+}
+
+// The packed copy and clover fields behind fullDslashQcu and cg_inverter.  The
+// reference reads the caller's gauge on every call (src/dslash.cu:73,
+// src/cg_inverter.cu:312-339), so a caller may change links in place between calls
+// (an HMC update, smearing) or pass a reused address.  The copy is therefore checked
+// against a fingerprint of the caller's buffer on every call -- one read of the gauge
+// -- and remade, with its clover fields and solver state dropped, when it differs.
+// QCU_DROPIN_GAUGE_CACHE=1 skips the check.  One copy is kept: a new pointer or
+// geometry replaces it.
 static GaugeField& dropin_gauge(void* gauge, const QcuParam* param, bool reload) {
   Context& c = ctx();
   QREQ(gauge != nullptr, "gauge pointer is NULL");
   QREQ(param != nullptr, "QcuParam pointer is NULL");
-  auto it = c.dropin.find(gauge);
-  bool same_geom = it != c.dropin.end() && memcmp(it->second->L, param->lattice_size, sizeof(int) * 4) == 0;
-  if (it == c.dropin.end() || reload || !same_geom) {
-    auto g = make_gauge_from_aos(gauge, param->lattice_size);
-    launch_gauge_reconstruct(g->aos.p, g->geo.vh, stream());
-    QHIP(hipGetLastError());
-    if (it != c.dropin.end()) drop_cached_batches(it->second.get());
-    c.dropin[gauge] = std::move(g);
+  require_aligned(gauge, "gauge");
+  validate_lattice(param->lattice_size);
+  Context::Dropin& d = c.dropin;
+  const bool same_geom = d.g && memcmp(d.g->L, param->lattice_size, sizeof(int) * 4) == 0;
+  const bool same = same_geom && d.ptr == gauge;
+  if (same && !reload && dropin_trust_pointer()) return *d.g;
+  const long vh = (long)param->lattice_size[0] * param->lattice_size[1] * param->lattice_size[2] *
+                  param->lattice_size[3] / 2;
+  const uint64_t fp = gauge_fingerprint(gauge, vh);
+  if (same && !reload && fp == d.fp) return *d.g;
+  if (d.g) {
+    QHIP(hipStreamSynchronize(stream()));
+    drop_cached_batches(d.g.get());
   }
-  return *c.dropin[gauge];
+  if (same_geom) {  // reuse the allocation: new links, no packed copies, no clover
+    GaugeField& g = *d.g;
+    for (auto& a : g.packed)
+      for (auto& b : a) b.release();
+    g.clovers.clear();
+    QHIP(hipMemcpyAsync(g.aos.p, gauge, g.aos.n, hipMemcpyDeviceToDevice, stream()));
+  } else {
+    d.g.reset();
+    d.g = make_gauge_from_aos(gauge, param->lattice_size);
+  }
+  launch_gauge_reconstruct(d.g->aos.p, d.g->geo.vh, stream());
+  QHIP(hipGetLastError());
+  d.ptr = gauge;
+  d.fp = fp;
+  return *d.g;
 }
 
 extern "C" void initGridSize(QcuGrid_t* grid, QcuParam* p_param, void* gauge, void* fermion_in, void* fermion_out) {
@@ -336,18 +385,23 @@ extern "C" void loadQcuGauge(void* gauge, QcuParam* param) {
   });
 }
 
+// One kernel on the caller's buffers: the gauge is read on every call, as the
+// reference's returned value (mpiDslashNaive) reads it, so no copy or fingerprint is
+// needed here.
 extern "C" void dslashQcu(void* fermion_out, void* fermion_in, void* gauge, QcuParam* param, int parity) {
   dropin("dslashQcu", [&] {
     ensure_init();
     QREQ(parity == 0 || parity == 1, "dslashQcu: parity must be 0 or 1");
-    QREQ(fermion_out && fermion_in, "dslashQcu: NULL spinor");
-    GaugeField& g = dropin_gauge(gauge, param, false);
-    long vh = g.geo.vh;
-    DevBuf* a = ctx().buf("dslash_in", spinor_bytes(F64, vh));
-    DevBuf* b = ctx().buf("dslash_out", spinor_bytes(F64, vh));
-    launch_spinor_from_aos(F64, fermion_in, 0, vec(a->p), vh, 1, stream());
-    hop(g, F64, 12, EPI_PLAIN, parity, 0, 1.0, vec(a->p), vec(b->p), vec(nullptr), nullptr, 1.0, 1, nullptr, stream());
-    launch_spinor_to_aos(F64, vec(b->p), fermion_out, 0, vh, 1, stream());
+    QREQ(fermion_out && fermion_in && gauge && param, "dslashQcu: NULL argument");
+    require_aligned(fermion_in, "fermion_in");
+    require_aligned(fermion_out, "fermion_out");
+    require_aligned(gauge, "gauge");
+    validate_lattice(param->lattice_size);
+    const Geom g = make_geom(param->lattice_size);
+    const char* i = static_cast<const char*>(fermion_in);
+    char* o = static_cast<char*>(fermion_out);
+    QREQ(o + g.vh * 192 <= i || i + g.vh * 192 <= o, "dslashQcu: fermion_out and fermion_in overlap");
+    launch_hop_iface(fermion_in, fermion_out, gauge, g, parity, 0, 1.0, 1.0, stream());
     QHIP(hipGetLastError());
     QHIP(hipStreamSynchronize(stream()));
   });
@@ -357,6 +411,8 @@ extern "C" void fullDslashQcu(void* fermion_out, void* fermion_in, void* gauge, 
   dropin("fullDslashQcu", [&] {
     ensure_init();
     QREQ(fermion_out && fermion_in, "fullDslashQcu: NULL spinor");
+    require_aligned(fermion_in, "fermion_in");
+    require_aligned(fermion_out, "fermion_out");
     GaugeField& g = dropin_gauge(gauge, param, false);
     qcuOperator op = {1.0, 1.0, 1, 12};
     apply_m_iface(g, op, fermion_out, fermion_in, dagger_flag ? 1 : 0);
@@ -374,6 +430,8 @@ extern "C" void cg_inverter(void* b_vector, void* x_vector, void* gauge, QcuPara
   int rc = guarded([&] {
     ensure_init();
     QREQ(b_vector && x_vector, "cg_inverter: NULL spinor");
+    require_aligned(b_vector, "b_vector");
+    require_aligned(x_vector, "x_vector");
     GaugeField& g = dropin_gauge(gauge, param, false);
     qcuOperator op = {1.0, 1.0, 1, 12};
     void* xs[1] = {x_vector};
@@ -470,7 +528,8 @@ int qcuFinalize(void) {
     if (!c.inited) return;
     QHIP(hipStreamSynchronize(c.stream));
     drop_cached_batches(nullptr);
-    c.dropin.clear();
+    c.dropin.g.reset();
+    c.dropin.ptr = nullptr;
     c.scratch.clear();
     QHIP(hipStreamDestroy(c.stream));
     c.stream = nullptr;

@@ -279,4 +279,69 @@ void launch_spinor_convert(int pin, Vec in, int pout, Vec out, long vh, int nrhs
 #undef QCU_SC
 }
 
+
+// ------------------------------------------------------------ fingerprint ---
+// A 64-bit fingerprint of a buffer: the sum (mod 2^64) over its 64-bit words w_i of
+// mix(w_i ^ (i * 0x9E3779B97F4A7C15)), mix being the splitmix64 finaliser (Steele,
+// Lea and Flood, OOPSLA 2014).  A sum of integers does not depend on the order it is
+// taken in, so the result is the same for any launch shape, and a change of any one
+// word changes it except with probability about 2^-64.  Used by the drop-in entry
+// points to notice a gauge the caller changed in place.
+__device__ __forceinline__ uint64_t fp_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    lo = __shfl_xor(lo, off, 64);
+    hi = __shfl_xor(hi, off, 64);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
+// partials[block] = sum over the block's grid-strided words
+__global__ void __launch_bounds__(256) fingerprint_kernel(const uint64_t* __restrict__ w, long n,
+                                                          uint64_t* __restrict__ partials) {
+  __shared__ uint64_t red[4];
+  constexpr uint64_t K = 0x9E3779B97F4A7C15ull;
+  uint64_t h = 0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(w);  // 16-byte loads
+  const long n2 = n / 2;
+#pragma unroll 4
+  for (long i = t0; i < n2; i += stride) {
+    const ulonglong2 q = w2[i];
+    h += fp_mix(q.x ^ ((uint64_t)(2 * i) * K)) + fp_mix(q.y ^ ((uint64_t)(2 * i + 1) * K));
+  }
+  if ((n & 1) && t0 == 0) h += fp_mix(w[n - 1] ^ ((uint64_t)(n - 1) * K));
+  h = wave_sum_u64(h);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void __launch_bounds__(256) fingerprint_sum_kernel(const uint64_t* __restrict__ partials, int n,
+                                                              uint64_t* __restrict__ out) {
+  __shared__ uint64_t red[4];
+  uint64_t h = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) h += partials[i];
+  h = wave_sum_u64(h);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = red[0] + red[1] + red[2] + red[3];
+}
+
+void launch_fingerprint(const void* buf, size_t bytes, uint64_t* scratch, hipStream_t st) {
+  const long n = (long)(bytes / 8);
+  hipLaunchKernelGGL(fingerprint_kernel, dim3(kFingerprintBlocks), dim3(256), 0, st,
+                     static_cast<const uint64_t*>(buf), n, scratch + 1);
+  hipLaunchKernelGGL(fingerprint_sum_kernel, dim3(1), dim3(256), 0, st, scratch + 1, kFingerprintBlocks, scratch);
+}
+
 }  // namespace qcu

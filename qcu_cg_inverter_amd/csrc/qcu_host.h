@@ -57,6 +57,10 @@ struct CloverField {
   double ck = 0.0;
   int nbad = 0;  // blocks inverted by the pivoted fallback, over the parities inverted so far
   long vh = 0;
+  // serving slots (of every server) solving with this field; a server frees the field
+  // only when this reaches zero, so servers that share a gauge never free a field
+  // another one still uses
+  int server_refs = 0;
   DevBuf A[3][2], Ainv[3][2];
   const void* a(int prec, int parity) const { return need(A[prec][parity]); }
   const void* ainv(int prec, int parity) const { return need(Ainv[prec][parity]); }
@@ -112,10 +116,15 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   std::recursive_mutex mu;
-  // state of the drop-in entry points
+  // state of the drop-in entry points: one resident copy, of the gauge last passed,
+  // with the fingerprint of the caller's buffer it was made from
   bool have_geom = false;
   int L[4] = {0, 0, 0, 0};
-  std::map<const void*, std::unique_ptr<GaugeField>> dropin;
+  struct Dropin {
+    const void* ptr = nullptr;
+    std::unique_ptr<GaugeField> g;
+    uint64_t fp = 0;
+  } dropin;
   qcuSolveParams default_solve;
   // scratch shared by calls (grow-only)
   std::map<std::string, std::unique_ptr<DevBuf>> scratch;

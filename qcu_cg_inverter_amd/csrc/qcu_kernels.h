@@ -21,6 +21,12 @@ __host__ __device__ inline long n_partials(long vh) { return (vh + 15) / 16; }
 // right-hand sides per wave (1, 2 or 4), rpt per thread (1..4)
 void launch_hop(int prec, int recon, int epi, const HopArgs& a, int block, int rb, int rpt, hipStream_t st);
 
+// the hop on the interface layouts (complex128 half spinors [Vh][4][3], gauge
+// [4][2][Vh][3][3] of which rows 0-1 are read): out = coef * D^s_{p,1-p} in, FP64,
+// straight from and to the caller's buffers (k_hop_iface.hip)
+void launch_hop_iface(const void* in, void* out, const void* gauge_aos, const Geom& g, int parity, int dagger,
+                      double tbc, double coef, hipStream_t st);
+
 // --------------------------------------------------------------- layout -----
 // interface spinor (complex128 [Vh][4][3]) <-> internal layout of `prec`
 void launch_spinor_from_aos(int prec, const void* aos, long aos_stride, Vec out, long vh, int nrhs, hipStream_t st);
@@ -34,6 +40,10 @@ void launch_gauge_reconstruct(void* aos, long vh, hipStream_t st);
 void launch_shift_storage(int kind, int to_soa, const void* src, void* dst, long vh, hipStream_t st);
 // lexicographic <-> even-odd, nc (12 or 9) complex per site, nfields fields of V sites
 void launch_lex_eo(int to_eo, const void* src, void* dst, const int L[4], int nc, int nfields, hipStream_t st);
+// 64-bit fingerprint of `bytes` (a multiple of 8) at buf, order-independent; scratch
+// holds kFingerprintBlocks + 1 words and the result lands in scratch[0]
+constexpr int kFingerprintBlocks = 1024;
+void launch_fingerprint(const void* buf, size_t bytes, uint64_t* scratch, hipStream_t st);
 // out = in (accumulate = 0) or out += in (accumulate = 1), any precisions
 void launch_spinor_convert(int pin, Vec in, int pout, Vec out, long vh, int nrhs, int accumulate, hipStream_t st);
 

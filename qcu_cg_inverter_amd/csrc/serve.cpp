@@ -65,7 +65,9 @@ struct Server {
   std::map<uint64_t, Request> reqs;
   uint64_t next_id = 1, next_tag = 1;
   std::vector<std::unique_ptr<BatchEntry>> batches;
-  std::map<std::pair<GaugeField*, double>, int> clover_refs;  // (gauge, kappa*csw) -> slots using it
+  // (gauge, kappa*csw) -> this server's slots using that clover field; the field's own
+  // CloverField::server_refs counts the slots of every server
+  std::map<std::pair<GaugeField*, double>, int> clover_refs;
   std::string error;
 
   void run();
@@ -74,9 +76,10 @@ struct Server {
   Admit admit_one(uint64_t id, Request& r, bool allow_alloc);
   void complete_rhs(Request& r, bool converged, int iterations, double true_residual);
   BatchEntry* find_batch(GaugeField* g, bool clover, int recon, int tbc);
-  BatchEntry* new_batch(GaugeField* g, bool clover, int recon, int tbc);
+  BatchEntry* new_batch(GaugeField* g, bool clover, int recon, int tbc, size_t extra);
+  void release_idle_batches(const BatchEntry* keep);
   size_t clover_cost(GaugeField* g, double ck);
-  void release_clover(GaugeField* g, double ck);
+  void release_clover(GaugeField* g, double ck, bool counted);
   bool round();
 };
 
@@ -85,18 +88,36 @@ size_t Server::clover_cost(GaugeField* g, double ck) {
   return solver_clover_bytes(cfg.precision, g->geo.vh);
 }
 
-void Server::release_clover(GaugeField* g, double ck) {
+// one slot of this server stops using the clover field (g, ck); counted: the slot had
+// been added to the field's server_refs.  The field is freed only when no slot of any
+// server uses it any more.
+void Server::release_clover(GaugeField* g, double ck, bool counted) {
   auto it = clover_refs.find({g, ck});
   if (it == clover_refs.end()) return;
+  auto c = g->clovers.find(ck);
+  if (counted && c != g->clovers.end()) c->second->server_refs--;
   if (--it->second > 0) return;
   clover_refs.erase(it);
-  const size_t per = solver_clover_bytes(cfg.precision, g->geo.vh);
-  auto c = g->clovers.find(ck);
-  if (c != g->clovers.end()) {
+  used -= solver_clover_bytes(cfg.precision, g->geo.vh);
+  if (c != g->clovers.end() && c->second->server_refs <= 0) {
     QHIP(hipStreamSynchronize(ctx().stream));
     g->clovers.erase(c);
   }
-  used -= per;
+}
+
+// give back the memory of batches with no active slot and no chunk in flight
+void Server::release_idle_batches(const BatchEntry* keep) {
+  for (size_t i = 0; i < batches.size();) {
+    BatchEntry* b = batches[i].get();
+    if (b != keep && b->batch->active() == 0 && !b->in_flight) {
+      used -= b->bytes;
+      QHIP(hipStreamSynchronize(b->stream));
+      QHIP(hipStreamDestroy(b->stream));
+      batches.erase(batches.begin() + i);
+    } else {
+      ++i;
+    }
+  }
 }
 
 BatchEntry* Server::find_batch(GaugeField* g, bool clover, int recon, int tbc) {
@@ -106,21 +127,13 @@ BatchEntry* Server::find_batch(GaugeField* g, bool clover, int recon, int tbc) {
   return nullptr;
 }
 
-BatchEntry* Server::new_batch(GaugeField* g, bool clover, int recon, int tbc) {
+// a new batch for (g, clover, recon, tbc) if it and `extra` bytes (a clover field its
+// first slot needs) fit the budget, after giving back idle batches if they do not
+BatchEntry* Server::new_batch(GaugeField* g, bool clover, int recon, int tbc, size_t extra) {
   size_t need = CgBatch::bytes_for(g->geo.vh, cfg.precision, cfg.max_batch);
-  if (used + need > budget) {
-    // give back idle batches first
-    for (size_t i = 0; i < batches.size();) {
-      if (batches[i]->batch->active() == 0 && !batches[i]->in_flight) {
-        used -= batches[i]->bytes;
-        QHIP(hipStreamSynchronize(batches[i]->stream));
-        QHIP(hipStreamDestroy(batches[i]->stream));
-        batches.erase(batches.begin() + i);
-      } else {
-        ++i;
-      }
-    }
-    if (used + need > budget) return nullptr;
+  if (used + need + extra > budget) {
+    release_idle_batches(nullptr);
+    if (used + need + extra > budget) return nullptr;
   }
   std::unique_ptr<BatchEntry> e(new BatchEntry);
   QHIP(hipStreamCreate(&e->stream));  // default priority
@@ -168,14 +181,17 @@ Server::Admit Server::admit_one(uint64_t id, Request& r, bool allow_alloc) {
   while (r.admitted < r.spec.nrhs) {
     BatchEntry* be = find_batch(g, clover, r.spec.recon, r.spec.t_boundary);
     if (be && be->in_flight) return ADMIT_WAIT;  // slots open at its next chunk boundary
-    if (clover) {
-      size_t cc = clover_cost(g, ck);
-      if (cc && (!allow_alloc || used + cc > budget)) return ADMIT_NO_MEMORY;
+    const size_t cc = clover ? clover_cost(g, ck) : 0;  // a clover field not yet in use here
+    if (cc && !allow_alloc) return ADMIT_NO_MEMORY;
+    if (!be) {
+      if (!allow_alloc) return ADMIT_NO_MEMORY;
+      be = new_batch(g, clover, r.spec.recon, r.spec.t_boundary, cc);
+      if (!be) return ADMIT_NO_MEMORY;
+    } else if (cc && used + cc > budget) {
+      release_idle_batches(be);  // idle batches of other gauges may hold the memory
+      if (used + cc > budget) return ADMIT_NO_MEMORY;
     }
-    if (!be && allow_alloc) be = new_batch(g, clover, r.spec.recon, r.spec.t_boundary);
-    if (!be) return ADMIT_NO_MEMORY;
     if (clover) {
-      size_t cc = clover_cost(g, ck);
       used += cc;
       if (used > peak) peak = used;
       clover_refs[{g, ck}]++;
@@ -198,8 +214,12 @@ Server::Admit Server::admit_one(uint64_t id, Request& r, bool allow_alloc) {
       // this right-hand side cannot be solved (e.g. a non-finite source): it fails
       // alone, the server and the other requests go on
       if (e.code == QCU_ERROR_HIP || e.code == QCU_ERROR_OUT_OF_MEMORY) throw;
-      if (clover) release_clover(g, ck);
+      if (clover) release_clover(g, ck, false);
       rejected = true;
+    }
+    if (clover && !rejected) {
+      auto c = g->clovers.find(ck);  // built by add()
+      if (c != g->clovers.end()) c->second->server_refs++;
     }
     std::lock_guard<std::mutex> lk(mu);
     if (!r.started) {
@@ -230,7 +250,7 @@ bool Server::round() {
       b->slot_req.erase(it);
       std::lock_guard<std::mutex> lk(mu);
       Request& r = reqs[id];
-      if (r.spec.csw != 0.0) release_clover(b->gauge, r.spec.kappa * r.spec.csw);
+      if (r.spec.csw != 0.0) release_clover(b->gauge, r.spec.kappa * r.spec.csw, true);
       complete_rhs(r, res.stats.converged != 0, res.stats.iterations, res.stats.true_residual);
     }
   }
@@ -389,7 +409,9 @@ int qcuServerSubmit(qcuServer h, const qcuRequest* req, uint64_t* id) {
     return QCU_ERROR_INVALID_VALUE;
   }
   GaugeField* g = reinterpret_cast<GaugeField*>(req->gauge);
+  // a batch of the request's gauge, plus the clover field its first slot builds
   size_t need = CgBatch::bytes_for(g->geo.vh, s->cfg.precision, s->cfg.max_batch);
+  if (req->csw != 0.0) need += solver_clover_bytes(s->cfg.precision, g->geo.vh);
   if (need > s->budget) {
     set_last_error(QCU_ERROR_ADMISSION, "request needs more device memory than the server's budget");
     return QCU_ERROR_ADMISSION;
@@ -486,9 +508,13 @@ int qcuServerDestroy(qcuServer h) {
       b->batch.reset();
       QHIP(hipStreamDestroy(b->stream));
     }
+    // this server's slots stop using their clover fields; a field another server
+    // still uses stays
     for (auto& kv : s->clover_refs) {
       auto c = kv.first.first->clovers.find(kv.first.second);
-      if (c != kv.first.first->clovers.end()) kv.first.first->clovers.erase(c);
+      if (c == kv.first.first->clovers.end()) continue;
+      c->second->server_refs -= kv.second;
+      if (c->second->server_refs <= 0) kv.first.first->clovers.erase(c);
     }
   } catch (const Error& e) {
     set_last_error(e.code, e.what());

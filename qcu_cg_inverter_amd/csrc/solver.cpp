@@ -128,23 +128,31 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
   maybe_autotune(g, P, op.recon, nrhs);  // QCU_TUNE=1: before any graph is captured
   CgBatch& batch = batch_for(g, P, op, nrhs, sp);
   batch.take_finished();
-  for (int r = 0; r < nrhs; ++r) {
-    SlotSpec s;
-    s.b = b[r];
-    s.seed = 0;
-    s.x = x[r];
-    s.kappa = op.kappa;
-    s.csw = op.csw;
-    s.tol = sp.tol;
-    s.max_iter = sp.max_iter;
-    s.tag = (uint64_t)r;
-    batch.add(s);
+  std::vector<SlotResult> res;
+  try {
+    for (int r = 0; r < nrhs; ++r) {
+      SlotSpec s;
+      s.b = b[r];
+      s.seed = 0;
+      s.x = x[r];
+      s.kappa = op.kappa;
+      s.csw = op.csw;
+      s.tol = sp.tol;
+      s.max_iter = sp.max_iter;
+      s.tag = (uint64_t)r;
+      batch.add(s);
+    }
+    while (batch.active() > 0) {
+      batch.launch_chunk();
+      batch.harvest();
+    }
+    res = batch.take_finished();
+  } catch (...) {
+    // slots added before the failure would stay active in the cached batch and make
+    // every later solve of this shape fail with "batch is full": drop the batch
+    drop_cached_batches(&g);
+    throw;
   }
-  while (batch.active() > 0) {
-    batch.launch_chunk();
-    batch.harvest();
-  }
-  std::vector<SlotResult> res = batch.take_finished();
   std::sort(res.begin(), res.end(), [](const SlotResult& a, const SlotResult& c) { return a.tag < c.tag; });
   bool all_conv = true, breakdown = false;
   qcuSolveStats agg;

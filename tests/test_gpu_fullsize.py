@@ -129,3 +129,28 @@ def test_config3_fp16_solve_true_residual_by_oracle(q):
     assert res <= 1e-10
     assert abs(res - st.true_residual) <= 1e-3 * st.true_residual
     g.destroy()
+
+
+def test_config2_fp64_and_mixed_solves_by_oracle(q):
+    """Configuration 2: 16^3x32 Wilson, kappa 0.124, eps 0.2, antiperiodic t, one
+    Gaussian source (the benchmark's seeds: gauge 1, source 2), tol 1e-10.  The plain
+    FP64 solve (18-real links) and the FP32-inner mixed solve with FP64 reliable
+    updates (reconstruct-12) both reach the tolerance, and the oracle's operator
+    recomputes each true residual.  Both solve the same system, so their solutions
+    agree to the tolerance times the condition number's share."""
+    L2 = (16, 16, 16, 32)
+    g = q.Gauge.random(L2, seed=1, eps=0.2)
+    gref = g.to_numpy()
+    b = q.source_gaussian(L2, seed=2)
+    xs = {}
+    for name, prec, recon in (("fp64", 0, 18), ("mixed_fp32", 1, 12)):
+        op = q.Operator(0.124, 0.0, -1, recon)
+        x, st = q.solve(g, op, b, q.default_solve_params(tol=1e-10, precision=prec))
+        assert st.converged and st.true_residual <= 1e-10, (name, st.as_dict())
+        r = b - orc.full_op(x, gref, L2, 0.124, s=1, tbc=-1.0, recon=recon)
+        res = np.linalg.norm(r) / np.linalg.norm(b)
+        assert res <= 1.001e-10, (name, res)
+        assert abs(res - st.true_residual) <= 1e-3 * st.true_residual, (name, res, st.true_residual)
+        xs[name] = x
+    assert rel(xs["mixed_fp32"], xs["fp64"]) < 1e-8
+    g.destroy()

@@ -181,6 +181,74 @@ def test_dropin_dslash_16x32(q):
     assert rel(out.to_numpy((2, vh, 4, 3)), ref) < FP64_TOL
 
 
+@pytest.mark.parametrize("L", [(2, 2, 2, 2), (6, 2, 2, 2), (8, 8, 8, 16)])
+def test_dropin_dslash_bits_equal_soa_hop(q, L):
+    """dslashQcu reads the interface layouts directly (one kernel); its bits equal the
+    SoA hop with reconstruct-12 on the same links, including half volumes that are not
+    a multiple of the 64-site wave (24 and 8 sites)."""
+    gref = orc.gauge_random(L, 17, 0.3)
+    prm = q.param(L)
+    gd = dev(gref)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    g = q.Gauge.from_numpy(gref, L)
+    vh = orc.half_volume(L)
+    v = orc.source_gauss(L, 3)
+    for parity in (0, 1):
+        inp = dev(v[1 - parity])
+        out = q.DeviceBuffer(vh * 192)
+        q.lib().dslashQcu(out.ptr, inp.ptr, gd.ptr, ctypes.byref(prm), parity)
+        assert q.lib().qcuGetLastError() == 0
+        soa = q.hop(g, v[1 - parity], parity, dagger=0, t_boundary=1, recon=12)
+        np.testing.assert_array_equal(out.to_numpy((vh, 4, 3)), soa)
+        ref = orc.hop(v[1 - parity], gref, L, parity, s=1, tbc=1.0, recon=12)
+        assert rel(out.to_numpy((vh, 4, 3)), ref) < FP64_TOL
+    g.destroy()
+
+
+def test_dropin_gauge_changed_in_place(q):
+    """The caller overwrites its gauge buffer between calls without loadQcuGauge (an
+    HMC update): dslashQcu, fullDslashQcu and cg_inverter must all apply the new
+    links, as the reference reads the gauge argument on every call."""
+    L = (8, 8, 8, 16)
+    g1 = orc.gauge_random(L, 31, 0.25)
+    g2 = orc.gauge_random(L, 32, 0.25)
+    prm = q.param(L)
+    gd = dev(g1)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    q.lib().loadQcuGauge(gd.ptr, ctypes.byref(prm))
+    vh = orc.half_volume(L)
+    v = orc.source_gauss(L, 6)
+    inp_h, inp_f = dev(v[1]), dev(v)
+    out_h, out_f = q.DeviceBuffer(vh * 192), q.DeviceBuffer(2 * vh * 192)
+    for gref in (g1, g2, g1):
+        gd.upload(gref)  # in place: same pointer, new links
+        q.lib().dslashQcu(out_h.ptr, inp_h.ptr, gd.ptr, ctypes.byref(prm), 0)
+        ref = orc.hop(v[1], gref, L, 0, s=1, tbc=1.0, recon=12)
+        assert rel(out_h.to_numpy((vh, 4, 3)), ref) < FP64_TOL
+        q.lib().fullDslashQcu(out_f.ptr, inp_f.ptr, gd.ptr, ctypes.byref(prm), 0)
+        A = orc.clover_fields(gref, L, 1.0, 1.0, recon=12)[0]
+        ref = orc.full_op(v, gref, L, 1.0, A=A, s=1, tbc=1.0, recon=12)
+        assert rel(out_f.to_numpy((2, vh, 4, 3)), ref) < FP64_TOL
+    assert q.lib().qcuGetLastError() == 0
+    # the solver's clover and packed links follow the change too
+    Ls = (4, 4, 4, 8)
+    h1, h2 = orc.gauge_random(Ls, 33, 0.25), orc.gauge_random(Ls, 34, 0.25)
+    prm = q.param(Ls)
+    gd = dev(h1)
+    q.lib().initGridSize(ctypes.byref(q.grid()), ctypes.byref(prm), gd.ptr, None, None)
+    b = orc.source_gauss(Ls, 9)
+    bd = dev(b)
+    vh = orc.half_volume(Ls)
+    xd = q.DeviceBuffer(2 * vh * 192)
+    for gref in (h1, h2):
+        gd.upload(gref)
+        q.lib().cg_inverter(bd.ptr, xd.ptr, gd.ptr, ctypes.byref(prm))
+        x = xd.to_numpy((2, vh, 4, 3))
+        A = orc.clover_fields(gref, Ls, 1.0, 1.0, recon=12)[0]
+        r = b - orc.full_op(x, gref, Ls, 1.0, A=A, s=1, tbc=1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-11
+
+
 def test_dropin_cg_inverter(q):
     L = (4, 4, 4, 8)
     gref = orc.gauge_random(L, 5, 0.25)
@@ -297,6 +365,27 @@ def test_error_paths(q):
     x, st = q.solve(g, op, b, q.default_solve_params(tol=1e-12, max_iter=3), raise_on_error=False)
     assert not st.converged and st.iterations <= 8
     assert q.lib().qcuGetLastError() == 4
+
+
+def test_batch_with_bad_source_leaves_no_stale_slots(q):
+    """A batched solve whose second source holds a NaN fails with code 6; the slot
+    already added for the first source must not stay behind, so the next clean solve
+    of the same shape (same gauge, operator, precision and batch size) converges."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=8, eps=0.2)
+    op = q.Operator(0.12, 1.0, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=1)
+    good = [orc.source_gauss(L, 21), orc.source_gauss(L, 22)]
+    bad = [good[0], good[1].copy()]
+    bad[1][1, 7, 0, 2] = np.nan
+    with pytest.raises(q.QcuError) as e:
+        q.solve_batch(g, op, bad, sp)
+    assert e.value.code == 6
+    xs, sts = q.solve_batch(g, op, good, sp)
+    assert all(s.converged and s.true_residual <= 1e-10 for s in sts)
+    xa, sa = q.solve(g, op, good[1], sp)
+    assert np.array_equal(xs[1], xa)
+    g.destroy()
 
 
 # --------------------------------------------------------------- batching ---

@@ -156,3 +156,155 @@ def test_server_stalled_request_not_starved(q):
     s.destroy()
     g1.destroy()
     g2.destroy()
+
+
+def test_server_config5_mix_all_volumes(q):
+    """Configuration 5's request mix under test: 8 resident gauges, two per volume over
+    8^4, 16^3x32, 24^3x48 and 32^3x64; Wilson and clover (c_sw = 1), kappa ~ U[0.11,
+    0.124], 1-12 right-hand sides, tol in {1e-8, 1e-10}; 40 requests submitted in
+    waves while earlier ones run.  Every request converges to its tolerance; for one
+    request per volume the oracle recomputes the true residual of its solutions; and a
+    32^3x64 clover request gives the same bits as when it is served alone."""
+    rng = np.random.default_rng(2025)
+    vols = [(8, 8, 8, 8), (16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
+    gauges = [q.Gauge.random(vols[i // 2], seed=900 + i, eps=0.2 + 0.05 * (i % 2)) for i in range(8)]
+
+    def draw(gi):
+        return {"g": gi, "kappa": float(rng.uniform(0.11, 0.124)), "csw": 1.0 if rng.random() < 0.5 else 0.0,
+                "nrhs": int(rng.integers(1, 13)), "tol": 1e-8 if rng.random() < 0.5 else 1e-10,
+                "seed": int(rng.integers(1 << 40))}
+
+    order = [int(i) for i in rng.permutation(8)] + [int(rng.integers(8)) for _ in range(32)]
+    reqs = [draw(gi) for gi in order]
+    # checked requests (explicit sources, solutions kept): the first of each volume,
+    # and a 32^3x64 clover one for the alone == mix comparison
+    checked = {}
+    for k, r in enumerate(reqs):
+        vi = r["g"] // 2
+        if vi not in checked:
+            r["nrhs"] = min(r["nrhs"], 2)
+            checked[vi] = k
+    big = {"g": 6, "kappa": 0.12, "csw": 1.0, "nrhs": 1, "tol": 1e-10, "seed": 77}
+    reqs.insert(12, big)
+    k_big = 12
+    checked = {vi: (k if k < k_big else k + 1) for vi, k in checked.items()}
+    bufs = {}
+
+    def buffers(k, r):
+        L = vols[r["g"] // 2]
+        vh = q.half_volume(L)
+        bs_h = [orc.source_gauss(L, r["seed"] + j) for j in range(r["nrhs"])]
+        return bs_h, [q.DeviceBuffer.from_numpy(b) for b in bs_h], [q.DeviceBuffer(2 * vh * 192) for _ in bs_h]
+
+    s = q.Server(precision=1, max_batch=12, chunk=8)
+    ids = []
+    for k, r in enumerate(reqs):
+        if k in checked.values() or k == k_big:
+            bufs[k] = buffers(k, r)
+            ids.append(s.submit(gauges[r["g"]], r["kappa"], r["csw"], r["tol"], r["nrhs"], xs=bufs[k][2],
+                                bs=bufs[k][1]))
+        else:
+            ids.append(s.submit(gauges[r["g"]], r["kappa"], r["csw"], r["tol"], r["nrhs"], seed=r["seed"]))
+        if k % 8 == 7:
+            time.sleep(0.05)  # the next wave arrives while this one runs
+    s.drain()
+    for rid, r in zip(ids, reqs):
+        st = s.poll(rid)
+        assert st.done and st.converged and st.true_residual <= r["tol"], (r, st.as_dict())
+    peak = s.usage()["peak_bytes"]
+    s.destroy()
+    assert peak > 0
+    # the oracle's residuals, one request per volume
+    for vi, k in sorted(checked.items()):
+        r = reqs[k]
+        L = vols[vi]
+        vh = q.half_volume(L)
+        gref = gauges[r["g"]].to_numpy()
+        A = orc.clover_fields(gref, L, r["kappa"], r["csw"], recon=12)[0] if r["csw"] else None
+        bs_h, _, xs = bufs[k]
+        for b, xd in zip(bs_h, xs):
+            x = xd.to_numpy((2, vh, 4, 3))
+            res = b - orc.full_op(x, gref, L, r["kappa"], A=A, s=1, tbc=-1.0, recon=12)
+            assert np.linalg.norm(res) / np.linalg.norm(b) <= 1.01 * r["tol"], (L, r)
+        del A, gref
+    # the 32^3x64 clover request alone gives the bits it got in the mix
+    L = vols[3]
+    vh = q.half_volume(L)
+    mix = bufs[k_big][2][0].to_numpy((2, vh, 4, 3))
+    s1 = q.Server(precision=1, max_batch=12, chunk=8)
+    xa = q.DeviceBuffer(2 * vh * 192)
+    st = s1.wait(s1.submit(gauges[6], 0.12, 1.0, 1e-10, 1, xs=[xa], bs=[bufs[k_big][1][0]]))
+    s1.destroy()
+    assert st.converged
+    assert np.array_equal(xa.to_numpy((2, vh, 4, 3)), mix)
+    for g in gauges:
+        g.destroy()
+
+
+def test_server_rejects_clover_beyond_budget(q):
+    """A budget that holds one batch but not the clover field a clover request also
+    needs: the request is rejected at submit (code 8) instead of waiting forever."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=61, eps=0.2)
+    probe = q.Server(precision=1, max_batch=4, chunk=2)
+    probe.wait(probe.submit(g, 0.12, 0.0, 1e-10, 1, seed=1))
+    one_batch = probe.usage()["bytes"]
+    probe.destroy()
+    s = q.Server(precision=1, max_batch=4, chunk=2, hbm_budget=one_batch + 4096)
+    with pytest.raises(q.QcuError) as e:
+        s.submit(g, 0.12, 1.0, 1e-10, 1, seed=2)
+    assert e.value.code == 8
+    rid = s.submit(g, 0.12, 0.0, 1e-10, 1, seed=3)  # a Wilson request still fits
+    assert s.wait(rid).converged
+    s.destroy()
+    g.destroy()
+
+
+def test_server_clover_after_idle_batch_of_other_gauge(q):
+    """A clover request finds an idle batch of its own kind with free slots, but its
+    new clover field fits only once an idle batch of another gauge gives its memory
+    back: that batch is released and the request admitted, not left waiting."""
+    L = (8, 8, 8, 8)
+    g1 = q.Gauge.random(L, seed=62, eps=0.2)
+    g2 = q.Gauge.random(L, seed=63, eps=0.2)
+    probe = q.Server(precision=1, max_batch=4, chunk=2)
+    probe.wait(probe.submit(g1, 0.12, 0.0, 1e-10, 1, seed=1))
+    one_batch = probe.usage()["bytes"]
+    probe.destroy()
+    s = q.Server(precision=1, max_batch=4, chunk=2, hbm_budget=2 * one_batch + 4096)
+    assert s.wait(s.submit(g1, 0.115, 1.0, 1e-10, 1, seed=2)).converged  # idle clover batch of g1
+    assert s.wait(s.submit(g2, 0.12, 0.0, 1e-10, 1, seed=3)).converged  # idle Wilson batch of g2
+    assert s.usage()["batches"] == 2
+    st = s.wait(s.submit(g1, 0.121, 1.0, 1e-10, 2, seed=4))  # a second kappa: a new clover field
+    assert st.converged and st.true_residual <= 1e-10
+    assert s.usage()["peak_bytes"] <= 2 * one_batch + 4096
+    s.destroy()
+    g1.destroy()
+    g2.destroy()
+
+
+def test_two_servers_share_a_clover_field(q):
+    """Two servers (FP32 and FP16 inner) solve clover requests on the same gauge and
+    kappa * c_sw, so their slots use one clover field.  One server finishes (and
+    releases its use) while the other still solves; the other's results stay right."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=64, eps=0.25)
+    vh = q.half_volume(L)
+    sa = q.Server(precision=1, max_batch=12, chunk=2)
+    sb = q.Server(precision=2, max_batch=12, chunk=2)
+    bs_h = [orc.source_gauss(L, 500 + j) for j in range(6)]
+    bs = [q.DeviceBuffer.from_numpy(b) for b in bs_h]
+    xs = [q.DeviceBuffer(2 * vh * 192) for _ in bs]
+    rb = sb.submit(g, 0.121, 1.0, 1e-10, 6, xs=xs, bs=bs)
+    ra = sa.submit(g, 0.121, 1.0, 1e-8, 1, seed=9)
+    assert sa.wait(ra).converged
+    sa.destroy()
+    st = sb.wait(rb)
+    sb.destroy()
+    assert st.converged and st.true_residual <= 1e-10
+    gref = g.to_numpy()
+    A = orc.clover_fields(gref, L, 0.121, 1.0, recon=12)[0]
+    for b, xd in zip(bs_h, xs):
+        r = b - orc.full_op(xd.to_numpy((2, vh, 4, 3)), gref, L, 0.121, A=A, s=1, tbc=-1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-10
+    g.destroy()
