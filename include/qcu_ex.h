@@ -122,6 +122,18 @@ int qcuHop(qcuGauge g, void* out, const void* in, int parity, int dagger, int t_
  * all of them); out[r] and in[r] are separate device buffers */
 int qcuHopBatch(qcuGauge g, int nrhs, void* const* out, const void* const* in, int parity, int dagger,
                 int t_boundary, int recon, int precision);
+/* qcuHop without device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and result as qcuHop, bit for
+ * bit, with the two internal staging fields in a static device array of 32 MiB that
+ * each device the library runs on holds until the process ends: a half volume whose two
+ * fields of `precision` exceed it (FP64 above 87,360 half sites; 16^3x32 fits) is
+ * rejected with QCU_ERROR_INVALID_VALUE.  It allocates nothing itself, so the gauge's
+ * packed links of this precision and recon must exist already (one qcuHop, qcuHopBatch
+ * or solve with them makes them), else QCU_ERROR_STATE.  Calls that overlap on one
+ * device share the array and race. */
+int qcuHopNoAlloc(qcuGauge g, void* out, const void* in, int parity, int dagger, int t_boundary, int recon,
+                  int precision);
 /* full lattice: out = (A - kappa D_s) in */
 int qcuApplyM(qcuGauge g, const qcuOperator* op, void* out, const void* in, int dagger);
 /* even half spinors: out = Mhat in = A_ee in - kappa^2 D_eo A_oo^-1 D_oe in (s = +1 or

@@ -127,6 +127,7 @@ PROTOTYPES = {
     "shiftGaugeStorage": (None, [_vp, _vp, _int, _int, _int, _int, _int]),
     "qcuHop": (_int, [_vp, _vp, _vp, _int, _int, _int, _int, _int]),
     "qcuHopBatch": (_int, [_vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _int, _int, _int, _int, _int]),
+    "qcuHopNoAlloc": (_int, [_vp, _vp, _vp, _int, _int, _int, _int, _int]),
     "qcuApplyM": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int]),
     "qcuApplyMhat": (_int, [_vp, ctypes.POINTER(Operator), _vp, _vp, _int, _int]),
     "qcuApplyComposite": (_int, [_vp, ctypes.POINTER(Operator), _int, _vp, _vp, _int]),

@@ -31,6 +31,7 @@ SOURCES = [
     "cg_batch.cpp",
     "tune.cpp",
     "serve.cpp",
+    "noalloc.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -104,6 +104,7 @@ void ensure_init() {
   if (n <= 0) throw Error(QCU_ERROR_HIP, "no HIP device");
   QHIP(hipSetDevice(c.device));
   QHIP(hipStreamCreate(&c.stream));  // default priority, blocking w.r.t. the null stream
+  noalloc_setup(c.device);           // device address of qcuHopNoAlloc's static scratch
   qcuDefaultSolveParams(&c.default_solve);
   c.default_solve.precision = QCU_PREC_DOUBLE;
   c.default_solve.tol = 1e-12;

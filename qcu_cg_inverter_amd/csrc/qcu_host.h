@@ -149,6 +149,9 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
 void maybe_autotune(GaugeField& g, int prec, int recon, int nrhs);
 
 void set_last_error(int code, const std::string& msg);
+// looks up the device address of qcuHopNoAlloc's static scratch (noalloc.cpp) on
+// `device`, which must be current
+void noalloc_setup(int device);
 
 // one hop launch on nrhs strided fields
 void hop(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, double tbc, Vec in, Vec out, Vec x,
