@@ -238,17 +238,20 @@ int qcuServerDestroy(qcuServer s);
  * gauge's volume and keeps the fastest for later launches; the eight fastest are
  * re-timed with each epilogue that reads x or a clover field, and solves use the
  * winner per epilogue.  shape (may be NULL; the plain epilogue's winner)
- * receives 11 ints: threads per block, wave groups per block sharing sites, right-hand
+ * receives 12 ints: threads per block, wave groups per block sharing sites, right-hand
  * sides per thread, XCD-contiguous block order (0/1), the z and t extents of the
  * site-walk tile (0 = storage order), right-hand side groups as the slowest block
  * index (1) or the fastest (0), the y extent of the walk tile (0 = all of y),
  * right-hand sides per wave (1, 2 or 4; a wave then covers 64, 32 or 16 sites),
- * site blocks walked by each block (sites per thread, 1..64), and the load policy of
- * links, clover and x (0 streaming, 1 cached).
+ * site blocks walked by each block (sites per thread, 1..64), the load policy of
+ * links, clover and x (0 streaming, 1 cached), and the kernel (0: the above; 1: every
+ * right-hand side in one block of 1, 2 or 4 waves per 64-site chunk, whose links are
+ * loaded once into LDS -- wave groups, right-hand sides per wave and per thread and
+ * group order then do not apply).
  * A tile that does not divide the lattice falls back to storage order.  QCU_TUNE=1 makes solves tune on first use;
  * QCU_TUNE_CACHE=<file> persists results across processes. */
 int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* shape, double* ms);
-/* Sets the launch shape (the 11 ints above, for every epilogue) used for (precision, recon, nrhs) on this
+/* Sets the launch shape (the 12 ints above, for every epilogue) used for (precision, recon, nrhs) on this
  * gauge's volume, e.g. one tuned earlier.  Every valid shape gives the same bits. */
 int qcuSetHopShape(qcuGauge g, int precision, int recon, int nrhs, const int* shape);
 

@@ -456,18 +456,19 @@ def solve_batch_device(g: Gauge, op: Operator, xs, bs, params: SolveParams | Non
 
 def tune_hop(g: Gauge, precision=PREC_DOUBLE, recon=18, nrhs=1):
     """Autotune the hopping kernel's launch shape; returns (shape dict, ms)."""
-    v = (ctypes.c_int * 11)()
+    v = (ctypes.c_int * len(SHAPE_KEYS))()
     ms = ctypes.c_double()
     check(lib().qcuTuneHop(g.handle, precision, recon, nrhs, v, ctypes.byref(ms)))
     return dict(zip(SHAPE_KEYS, list(v))), ms.value
 
 
-SHAPE_KEYS = ("block", "rb", "rpt", "xcd", "tile_z", "tile_t", "group_major", "tile_y", "rpw", "spt", "cached")
+SHAPE_KEYS = ("block", "rb", "rpt", "xcd", "tile_z", "tile_t", "group_major", "tile_y", "rpw", "spt", "cached",
+              "lds")
 
 
 def set_hop_shape(g: Gauge, precision, recon, nrhs, shape: dict):
     """Pin the hopping kernel's launch shape for (precision, recon, nrhs) on g's volume."""
-    v = (ctypes.c_int * 11)(*[int(shape.get(k, 1 if k in ("rpw", "spt") else 0)) for k in SHAPE_KEYS])
+    v = (ctypes.c_int * len(SHAPE_KEYS))(*[int(shape.get(k, 1 if k in ("rpw", "spt") else 0)) for k in SHAPE_KEYS])
     check(lib().qcuSetHopShape(g.handle, precision, recon, nrhs, v))
 
 

@@ -225,14 +225,12 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs, int epi) {
   it = ctx().tune.find(tune_key(prec, recon, vh, nrhs, EPI_PLAIN));
   if (it != ctx().tune.end()) return it->second;
   Tune t;
-  if (nrhs > 1) {  // waves of 16 sites x 4 right-hand sides (or 32 x 2) sharing link loads,
-    // one right-hand side per thread (several per thread need more registers than the
-    // occupancy they cost is worth), up to 4 such waves per block on the same sites
-    t.rpw = nrhs >= 4 ? 4 : 2;
-    t.rb = (nrhs + t.rpw - 1) / t.rpw;
-    if (t.rb > 4) t.rb = 4;
-    t.rpt = 1;
-    t.block = 64 * t.rb;
+  if (nrhs > 1) {  // every right-hand side in one block per 64-site chunk, its links
+    // staged in LDS once (hop_lds_kernel; on 32^3x64 with 12 right-hand sides 22-29 %
+    // faster than the best shape sharing link loads within a wave), 4 waves per block
+    t.lds = 1;
+    t.block = nrhs >= 4 ? 256 : 128;
+    t.xcd = 0;
     t.tz = t.tt = 4;  // short neighbour reuse distances (ignored where tiles do not fit)
   }
   return t;
@@ -284,6 +282,7 @@ void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, 
   a.rpw = t.rpw;
   a.spt = t.spt;
   a.cached = t.cached;
+  a.lds = t.lds;
   a.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }

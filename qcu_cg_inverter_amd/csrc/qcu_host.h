@@ -96,7 +96,9 @@ struct GaugeField {
 // whether the right-hand side group is the slowest block index (gm) or the fastest,
 // `rpw` right-hand sides per wave (lanes of a wave sharing sites), `spt` site blocks
 // each block walks (sites per thread), and whether links / clover / x use ordinary
-// (cached) loads instead of streaming ones
+// (cached) loads instead of streaming ones; `lds` selects the kernel that stages a
+// chunk's links in LDS for all right-hand sides (block = 64 x waves; rb, rpw, rpt and
+// gm do not apply)
 struct Tune {
   int block = 256;
   int rb = 1;
@@ -109,6 +111,7 @@ struct Tune {
   int rpw = 1;
   int spt = 1;
   int cached = 0;
+  int lds = 0;  // 1: every right-hand side in one block per 64-site chunk, its links staged in LDS
 };
 
 struct Context {

@@ -2,7 +2,8 @@
 //
 // For one (precision, link storage, half volume, right-hand sides) it times every
 // candidate shape -- block size, wave groups per block sharing sites, right-hand
-// sides per wave, group order, XCD-contiguous block order, walk tile -- on scratch
+// sides per wave, group order, XCD-contiguous block order, walk tile, and the kernel
+// that stages a site chunk's links in LDS for every right-hand side -- on scratch
 // fields with hipEvents and the plain epilogue, keeps the fastest, and re-times the
 // eight fastest, and the two fastest that take every right-hand side in one block per
 // site range, with each epilogue that reads x or a clover field (those add loads that
@@ -35,8 +36,8 @@ void load_tune_cache() {
   while (fgets(line, sizeof line, f)) {
     Tune t;
     double ms;
-    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz,
-               &t.tt, &t.gm, &t.ty, &t.rpw, &t.spt, &t.cached, &ms) == 13)
+    if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz,
+               &t.tt, &t.gm, &t.ty, &t.rpw, &t.spt, &t.cached, &t.lds, &ms) == 14)
       ctx().tune[key] = t;
   }
   fclose(f);
@@ -47,8 +48,8 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt,
-          t.gm, t.ty, t.rpw, t.spt, t.cached, ms);
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt,
+          t.gm, t.ty, t.rpw, t.spt, t.cached, t.lds, ms);
   fclose(f);
 }
 
@@ -84,6 +85,24 @@ static std::vector<Tune> candidates(int prec, int nrhs) {
               out.push_back(t);
             }
           }
+        }
+  // the LDS-staged kernel: every right-hand side in one block of 1, 2 or 4 waves
+  if (nrhs > 1)
+    for (auto& tl : tiles)
+      for (int xcd = 0; xcd <= 1; ++xcd)
+        for (int nw = 1; nw <= 4; nw *= 2) {
+          if (nw > nrhs) continue;
+          Tune t;
+          t.block = 64 * nw;
+          t.rb = 1;
+          t.rpt = 1;
+          t.rpw = 1;
+          t.xcd = xcd;
+          t.ty = tl[0];
+          t.tz = tl[1];
+          t.tt = tl[2];
+          t.lds = 1;
+          out.push_back(t);
         }
   return out;
 }
@@ -131,6 +150,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
     args.rpw = t.rpw;
     args.spt = t.spt;
     args.cached = t.cached;
+    args.lds = t.lds;
     args.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
     for (int w = 0; w < 2; ++w) launch_hop(prec, recon, epi, args, t.block, t.rb, t.rpt, stream());
     QHIP(hipEventRecord(e0, stream()));
@@ -176,7 +196,8 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   std::vector<Tune> epi_shapes;
   auto same = [](const Tune& x, const Tune& y) {
     return x.block == y.block && x.rb == y.rb && x.rpt == y.rpt && x.xcd == y.xcd && x.tz == y.tz &&
-           x.tt == y.tt && x.gm == y.gm && x.ty == y.ty && x.rpw == y.rpw && x.spt == y.spt && x.cached == y.cached;
+           x.tt == y.tt && x.gm == y.gm && x.ty == y.ty && x.rpw == y.rpw && x.spt == y.spt && x.cached == y.cached &&
+           x.lds == y.lds;
   };
   auto add_shape = [&](const Tune& t) {
     for (const Tune& u : epi_shapes)
@@ -187,7 +208,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   int one_group = 0;
   for (size_t k = 0; k < timed.size() && one_group < 2; ++k) {
     Tune t = timed[k].second;
-    if (t.rb * t.rpw * t.rpt < nrhs || t.spt != 1) continue;
+    if ((!t.lds && t.rb * t.rpw * t.rpt < nrhs) || t.spt != 1) continue;
     t.cached = 0;
     add_shape(t);
     t.cached = 1;
@@ -247,6 +268,7 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* s
       shape[8] = t.rpw;
       shape[9] = t.spt;
       shape[10] = t.cached;
+      shape[11] = t.lds;
     }
     return QCU_SUCCESS;
   } catch (const Error& e) {
@@ -276,6 +298,9 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     t.rpw = shape[8];
     t.spt = shape[9];
     t.cached = shape[10];
+    t.lds = shape[11];
+    QREQ(t.lds == 0 || t.lds == 1, "lds must be 0 or 1");
+    QREQ(!t.lds || t.block == 64 || t.block == 128 || t.block == 256, "the LDS kernel takes 1, 2 or 4 waves per block");
     QREQ(t.block == 64 || t.block == 128 || t.block == 192 || t.block == 256, "block must be 64, 128, 192 or 256");
     QREQ(t.rpw == 1 || t.rpw == 2 || t.rpw == 4, "right-hand sides per wave must be 1, 2 or 4");
     QREQ(t.spt >= 1 && t.spt <= 64, "site blocks per block must be 1..64");

@@ -418,7 +418,8 @@ def test_tuned_shape_keeps_bits(q, prec):
 
 
 @pytest.mark.parametrize("shape", [None, (256, 4, 3, 1, 0, 0, 0), (128, 2, 2, 1, 0, 0, 1),
-                                   (192, 3, 1, 1, 0, 0, 0, 0, 4, 2, 1), (64, 1, 1, 0, 0, 0, 1, 0, 2, 4, 0)])
+                                   (192, 3, 1, 1, 0, 0, 0, 0, 4, 2, 1), (64, 1, 1, 0, 0, 0, 1, 0, 2, 4, 0),
+                                   (256, 1, 1, 1, 4, 4, 0, 0, 1, 1, 1, 1), (128, 1, 1, 0, 0, 0, 0, 0, 1, 2, 0, 1)])
 def test_batch12_clover_mixed(q, shape):
     """12 right-hand sides in one batched FP32-inner clover solve (default launch
     shape, or several right-hand sides per thread in either group order): every one
@@ -444,16 +445,20 @@ def test_batch12_clover_mixed(q, shape):
 def test_hop_shapes_keep_bits(q, prec):
     """Every launch shape of a 12-RHS hop -- block size, wave groups sharing sites,
     right-hand sides per thread, XCD order, walk tile, right-hand side groups as the
-    fastest or the slowest block index -- gives the bits of single launches."""
+    fastest or the slowest block index, the kernel with links staged in LDS -- gives
+    the bits of single launches."""
     L = (16, 16, 8, 16)  # 128-site planes: whole-plane and 8-row (y, z, t) walk tiles apply
     g = q.Gauge.random(L, seed=34, eps=0.3)
     vs = [orc.source_gauss(L, 80 + r)[1] for r in range(12)]
     singles = [q.hop(g, v, 0, dagger=0, t_boundary=-1, recon=12, precision=prec) for v in vs]
-    # (block, rb, rpt, xcd, tile_z, tile_t, group_major, tile_y, rpw, spt, cached)
+    # (block, rb, rpt, xcd, tile_z, tile_t, group_major, tile_y, rpw, spt, cached[, lds])
     shapes = [(256, 4, 1, 1, 4, 4, 1, 0, 1, 1, 0), (128, 2, 3, 1, 4, 4, 1, 8, 1, 2, 1), (64, 1, 2, 0, 0, 0, 1, 0, 1, 1, 0),
               (256, 2, 1, 1, 8, 8, 0, 8, 1, 4, 1), (128, 1, 4, 0, 4, 4, 0, 0, 1, 1, 0), (256, 4, 1, 1, 2, 4, 0, 8, 1, 3, 0),
               (192, 3, 1, 1, 4, 4, 0, 0, 4, 1, 1), (64, 1, 1, 1, 0, 0, 1, 0, 4, 2, 0), (128, 2, 1, 0, 4, 4, 1, 8, 2, 4, 1),
-              (256, 4, 1, 1, 0, 0, 0, 0, 4, 1, 0), (256, 2, 1, 1, 8, 8, 1, 0, 2, 2, 1)]
+              (256, 4, 1, 1, 0, 0, 0, 0, 4, 1, 0), (256, 2, 1, 1, 8, 8, 1, 0, 2, 2, 1),
+              # the LDS-staged kernel (last entry 1): 4, 2 and 1 waves per block
+              (256, 1, 1, 1, 4, 4, 0, 8, 1, 1, 0, 1), (128, 1, 1, 0, 0, 0, 0, 0, 1, 2, 1, 1),
+              (64, 1, 1, 1, 8, 8, 0, 0, 1, 3, 0, 1), (256, 1, 1, 0, 2, 4, 0, 8, 1, 1, 1, 1)]
     for sh in shapes:
         shape = dict(zip(q.SHAPE_KEYS, sh))
         if prec == 0:
