@@ -227,11 +227,13 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs, int epi) {
   Tune t;
   if (nrhs > 1) {  // every right-hand side in one block per 64-site chunk, its links
     // staged in LDS once (hop_lds_kernel; on 32^3x64 with 12 right-hand sides 22-29 %
-    // faster than the best shape sharing link loads within a wave), 4 waves per block
+    // faster than the best shape sharing link loads within a wave), 4 waves per block,
+    // blocks round-robin over the XCDs
     t.lds = 1;
     t.block = nrhs >= 4 ? 256 : 128;
     t.xcd = 0;
-    t.tz = t.tt = 4;  // short neighbour reuse distances (ignored where tiles do not fit)
+    t.tz = -1;  // tiles of all of z by 8 t-planes (ignored where they do not fit)
+    t.tt = 8;
   }
   return t;
 }
@@ -283,7 +285,7 @@ void hop_v(GaugeField& g, int prec, int recon, int epi, int parity, int dagger, 
   a.spt = t.spt;
   a.cached = t.cached;
   a.lds = t.lds;
-  a.walk = make_walk(g.geo, t.ty, t.tz, t.tt);
+  a.walk = make_walk(g.geo, t.ty, t.tz < 0 ? g.geo.L[2] : t.tz, t.tt);
   launch_hop(prec, recon, epi, a, t.block, t.rb, t.rpt, st);
 }
 

@@ -104,7 +104,7 @@ struct Tune {
   int rb = 1;
   int rpt = 1;
   int xcd = 1;
-  int tz = 0;
+  int tz = 0;  // -1: all of z
   int tt = 0;
   int gm = 0;
   int ty = 0;

@@ -10,6 +10,7 @@
 // can favour another shape).  Results live in the context's table and, when
 // QCU_TUNE_CACHE names a file, are appended there and reloaded by later processes.
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -56,7 +57,7 @@ static void append_cache(const std::string& key, const Tune& t, double ms) {
 // Several right-hand sides per thread (qcuSetHopShape) spill in FP64 and drop FP32 /
 // FP16 to one wave per SIMD, so the search keeps one per thread and varies how many
 // share a wave (rpw) and a block (rb).
-static std::vector<Tune> candidates(int prec, int nrhs) {
+static std::vector<Tune> candidates(const Geom& geo, int prec, int nrhs) {
   (void)prec;
   std::vector<Tune> out;
   const int blocks[] = {64, 128, 192, 256};
@@ -86,9 +87,16 @@ static std::vector<Tune> candidates(int prec, int nrhs) {
             }
           }
         }
-  // the LDS-staged kernel: every right-hand side in one block of 1, 2 or 4 waves
+  // the LDS-staged kernel: every right-hand side in one block of 1, 2 or 4 waves, also
+  // with tiles of all of z by 4 or 8 t-planes (on 32^3x64 with 12 FP32 right-hand
+  // sides, XCD round-robin with tiles of 32 z by 8 t was the fastest of ten shapes
+  // tried: 784 us against 847 us for 4 x 4 tiles)
+  std::vector<std::array<int, 3>> ltiles;
+  for (auto& tl : tiles) ltiles.push_back({tl[0], tl[1], tl[2]});
+  ltiles.push_back({0, geo.L[2], 8});
+  ltiles.push_back({0, geo.L[2], 4});
   if (nrhs > 1)
-    for (auto& tl : tiles)
+    for (auto& tl : ltiles)
       for (int xcd = 0; xcd <= 1; ++xcd)
         for (int nw = 1; nw <= 4; nw *= 2) {
           if (nw > nrhs) continue;
@@ -167,7 +175,7 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
   };
   // every candidate with the plain epilogue ...
   std::vector<std::pair<double, Tune>> timed;
-  for (const Tune& t : candidates(prec, nrhs)) {
+  for (const Tune& t : candidates(g.geo, prec, nrhs)) {
     if (t.tz && !make_walk(g.geo, t.ty, t.tz, t.tt).tile_z) continue;  // tile does not fit this lattice
     timed.push_back({time_shape(t, EPI_PLAIN), t});
   }
