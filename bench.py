@@ -24,6 +24,7 @@ The same JSON line also carries:
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
 import argparse
+import hashlib
 import json
 import os
 import platform
@@ -49,6 +50,49 @@ def hop_bytes(prec, recon, nrhs=1):
     sides: 8 links shared by all, then per right-hand side 8 neighbour spinors read
     and one written."""
     return 8 * LINK_BYTES[(prec, recon)] + nrhs * 9 * SPINOR_BYTES[prec]
+
+
+def compulsory_bytes(prec, recon, nrhs=1):
+    """HBM bytes per output half-site that one launch cannot avoid: each of the 8
+    links once, and per right-hand side one read of the input spinor and one write of
+    the output (the 8 neighbour reads of a spinor can come from cache after the
+    first).  FP64 recon-18: 8 x 144 + 2 x 192 = 1536 B."""
+    return 8 * LINK_BYTES[(prec, recon)] + nrhs * 2 * SPINOR_BYTES[prec]
+
+
+# sources that fix the hop kernel's code: a PMC traffic figure measured on other
+# sources is stale and is not reported
+HOP_SOURCES = ("k_hop.inc", "k_hop.hip", "qcu_fields.h", "qcu_common.h", "qcu_kernels.h")
+
+
+def hop_source_hash():
+    h = hashlib.sha256()
+    for n in HOP_SOURCES:
+        with open(os.path.join(ROOT, "qcu_cg_inverter_amd", "csrc", n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def roofline(kernel, prec, recon, vh, ms, traffic_name, nrhs=1):
+    """The hop against the 8 TB/s HBM peak.  `achieved` counts the compulsory bytes of
+    one launch (compulsory_bytes x Vh), so a correct kernel cannot exceed the peak;
+    `achieved_algorithmic` counts the 8 neighbour reads of every spinor as well
+    (hop_bytes, the per-site figure the metric is quoted in), which the caches serve
+    after the first.  `traffic` is the HBM bytes per launch read off rocprofv3 PMC
+    counters (FETCH_SIZE and WRITE_SIZE, scaled by what they read for a copy of known
+    size) for these kernel sources, or None."""
+    comp = compulsory_bytes(prec, recon, nrhs) * vh
+    alg = hop_bytes(prec, recon, nrhs) * vh
+    ach = comp / (ms * 1e-3) / 1e9
+    t = measured_traffic(traffic_name)
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": t, "kernel": kernel,
+            "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": comp,
+            "bytes_basis": f"compulsory: {compulsory_bytes(prec, recon, nrhs)} B per half site",
+            "traffic_GBps": round(t / (ms * 1e-3) / 1e9, 1) if t else None,
+            "achieved_algorithmic": round(alg / (ms * 1e-3) / 1e9, 1), "algorithmic_bytes_per_launch": alg,
+            "frac_algorithmic": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel_sources_sha16": hop_source_hash()}
 
 
 def timed(fn):
@@ -78,6 +122,37 @@ def multi_rhs_section(q):
     return res
 
 
+def config2_section(q, g=None):
+    """16^3x32 Wilson (kappa = 0.124, eps = 0.2, antiperiodic t), one Gaussian source,
+    true residual 1e-10: plain FP64 CG against FP32- and FP16-inner CG with FP64
+    reliable updates."""
+    own = g is None
+    if own:
+        g = q.Gauge.random(L_MAIN, seed=1, eps=0.2)
+    b = q.source_gaussian(L_MAIN, seed=2)
+    op = q.Operator(0.124, 0.0, -1, 12)
+    cg = {}
+    for name, prec in (("fp64", 0), ("mixed_fp32", 1), ("mixed_fp16", 2)):
+        sp = q.default_solve_params(tol=1e-10, precision=prec)
+        if prec == 0:
+            op64 = q.Operator(0.124, 0.0, -1, 18)
+            q.solve(g, op64, b, sp)  # warm (graph capture paths, buffers)
+            x, st = q.solve(g, op64, b, sp)
+        else:
+            q.solve(g, op, b, sp)
+            x, st = q.solve(g, op, b, sp)
+        cg[name] = {"seconds": round(st.seconds, 5), "iterations": st.iterations,
+                    "reliable_updates": st.reliable_updates, "true_residual": st.true_residual,
+                    "converged": bool(st.converged)}
+        log(f"cg {name}: {st.seconds*1e3:.2f} ms, {st.iterations} it, true res {st.true_residual:.2e}")
+    cg["speedup_mixed_fp32_vs_fp64"] = round(cg["fp64"]["seconds"] / cg["mixed_fp32"]["seconds"], 3)
+    cg["speedup_mixed_fp16_vs_fp64"] = round(cg["fp64"]["seconds"] / cg["mixed_fp16"]["seconds"], 3)
+    cg["problem"] = "16^3x32 Wilson, kappa=0.124, eps=0.2, antiperiodic t, 1 Gaussian source, tol 1e-10"
+    if own:
+        g.destroy()
+    return cg
+
+
 def config3_section(q):
     """24^3x48 Wilson-clover c_sw=1, kappa=0.122, eps=0.25, 1 Gaussian source, tol 1e-10:
     FP16 (+ per-site norm) inner with FP64 reliable updates vs plain FP64."""
@@ -99,7 +174,7 @@ def config3_section(q):
     return out
 
 
-def serving_section(q, n_requests=240, seed=2024):
+def serving_section(q, n_requests=2000, seed=2024):
     """Poisson stream of solve requests over 8 resident gauges (2 per volume in
     {8^4, 16^3x32, 24^3x48, 32^3x64}); each request draws kappa ~ U[0.11, 0.124],
     Wilson or clover (c_sw = 1.0), 1-12 Gaussian right-hand sides and tol in
@@ -107,7 +182,9 @@ def serving_section(q, n_requests=240, seed=2024):
     0.8x capacity through the continuous-batching server (FP32 inner, FP64 reliable
     updates).  The baseline is the plain FP64 solve, one right-hand side at a time:
     one solve per request class is timed and the whole stream is replayed through a
-    FIFO queue with the same arrival times."""
+    FIFO queue with the same arrival times; that model is checked against a measured
+    one-at-a-time FP64 run of a seeded subsample of 40 of the stream's requests, each
+    at its own kappa, and its error is reported."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -223,7 +300,47 @@ def serving_section(q, n_requests=240, seed=2024):
         "p50_ms": round(float(np.percentile(blat, 50)), 2), "p99_ms": round(float(np.percentile(blat, 99)), 2),
         "note": "FIFO replay of the same arrivals; per-request service = nrhs x the measured plain FP64 solve "
                 "time of its class (gauge, Wilson/clover, tol) at kappa 0.117"}
+    # the model checked against a measured run: a seeded subsample of the stream's
+    # requests solved in plain FP64, one right-hand side at a time, each at its own
+    # kappa, c_sw, tol and sources
+    sub_n = min(40, n_requests)
+    sub = sorted(rng.choice(n_requests, size=sub_n, replace=False).tolist())
+    meas, model = [], []
+    log(f"serving: measuring the plain FP64 one-at-a-time baseline on {sub_n} of the stream's requests")
+    for k in sub:
+        r = reqs[k]
+        L = vols[r["g"] // 2]
+        sp.tol = r["tol"]
+        op = q.Operator(r["kappa"], r["csw"], -1, 12)
+        x = q.DeviceBuffer(2 * q.half_volume(L) * 192)
+        bsrc = [q.source_gaussian_device(L, r["seed"] + j) for j in range(r["nrhs"])]
+        t1 = time.perf_counter()
+        for b in bsrc:
+            st = q.solve_batch_device(gauges[r["g"]], op, [x], [b], sp)[0]
+            assert st.converged, ("baseline solve did not converge", r)
+        meas.append(time.perf_counter() - t1)
+        model.append(service[k])
+        for b in bsrc:
+            b.free()
+        x.free()
+    meas, model = np.array(meas), np.array(model)
+    relerr = (model - meas) / meas
+    out["baseline_fp64_one_at_a_time"]["measured_subsample"] = {
+        "requests": sub_n, "rhs": int(sum(reqs[k]["nrhs"] for k in sub)),
+        "measured_s": round(float(meas.sum()), 3), "modelled_s": round(float(model.sum()), 3),
+        "model_error_total": round(float(model.sum() / meas.sum() - 1.0), 4),
+        "model_error_per_request_median": round(float(np.median(relerr)), 4),
+        "model_error_per_request_max_abs": round(float(np.abs(relerr).max()), 4),
+        "solves_per_s_measured": round(sum(reqs[k]["nrhs"] for k in sub) / float(meas.sum()), 2),
+        "note": "each sampled request's right-hand sides solved one at a time in plain FP64 at the request's "
+                "own kappa, c_sw, tol and seeded sources; model = nrhs x its class time"}
+    # the baseline rate with the model's service times rescaled by its measured error
+    out["baseline_fp64_one_at_a_time"]["solves_per_s_corrected"] = round(
+        nrhs_total / (sum(service) * float(meas.sum() / model.sum())), 2)
+    log(f"serving: baseline model error over the subsample {100 * (model.sum() / meas.sum() - 1):.1f} %")
     out["speedup_solves_per_s"] = round(out["solves_per_s"] / out["baseline_fp64_one_at_a_time"]["solves_per_s"], 3)
+    out["speedup_solves_per_s_vs_corrected"] = round(
+        out["solves_per_s"] / out["baseline_fp64_one_at_a_time"]["solves_per_s_corrected"], 3)
     log(f"serving: {out['solves_per_s']} solves/s p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms; "
         f"fp64 one-at-a-time {out['baseline_fp64_one_at_a_time']['solves_per_s']} solves/s")
     for g in gauges:
@@ -336,14 +453,18 @@ def cpu_info():
 
 
 def measured_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 PMC counters (committed under
-    profiles/), or None when no measurement for this kernel/workload exists."""
+    """HBM bytes per launch measured with rocprofv3 PMC counters (profiles/traffic.json,
+    each entry stamped with the hop sources' hash), or None when there is no
+    measurement of this kernel and workload on the current sources."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(name)
+            e = json.load(f).get(name)
     except (OSError, ValueError):
         return None
+    if not isinstance(e, dict) or e.get("kernel_sources_sha16") != hop_source_hash():
+        return None
+    return e.get("bytes")
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -471,11 +592,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--quick", action="store_true", help="skip the large lattices and the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=240,
+    ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests in the Poisson serving stream (2000 for the full configuration; 0 skips)")
     ap.add_argument("--kernels-12rhs", action="store_true",
-                    help="profiler driver: copy kernel, then the 32^3x64 FP32 hop with 1 and 12 right-hand sides")
-    ap.add_argument("--section", choices=["config1", "config3", "config4", "serving", "dslash_12rhs"],
+                    help="profiler driver: copy kernel, then the 32^3x64 FP32 and FP16 hops with 1 and 12 "
+                         "right-hand sides")
+    ap.add_argument("--section", choices=["config1", "config2", "config3", "config4", "serving", "dslash_12rhs"],
                     help="profiler driver: run one section only and print its JSON")
     ap.add_argument("--kernels-large", action="store_true",
                     help="profiler driver: 1 GiB copy, then the FP64 recon-18 and recon-12 hops on 32^3x64")
@@ -486,8 +608,6 @@ def main():
         print(f"bench.py: this build keeps the lattice on one GPU; --gpus {args.gpus} is not supported "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
-
-    import numpy as np
 
     import qcu_cg_inverter_amd as q
 
@@ -512,35 +632,27 @@ def main():
             log(f"kernels-large: f64 r{recon} 32^3x64 {ms*1e3:.1f} us/launch")
         return 0
     if args.section:
-        fn = {"config1": config1_gpu, "config3": config3_section, "config4": config4_section,
+        fn = {"config1": config1_gpu, "config2": config2_section, "config3": config3_section,
+              "config4": config4_section,
               "serving": lambda q_: serving_section(q_, args.serving_requests),
               "dslash_12rhs": multi_rhs_section}[args.section]
         print(json.dumps({args.section: fn(q)}))
         return 0
     if args.kernels_12rhs:
-        # profiler driver: 1 GiB copy x5, then the 32^3x64 FP32 recon-12 hop with 1
-        # right-hand side, with 12 in the autotuned shape, and with 12 in that shape
-        # with the other right-hand side group order.  With QCU_TUNE_CACHE naming an
-        # existing file the shapes come from it, so profiled runs launch no tuning.
+        # profiler driver: 1 GiB copy x5, then on 32^3x64 the FP32 and the FP16 recon-12
+        # hops with 1 right-hand side and with 12 in the autotuned shape.  With
+        # QCU_TUNE_CACHE naming an existing file the shapes come from it, so profiled
+        # runs launch no tuning.
         q.bench_copy(1 << 30, iters=5)
         gk = q.Gauge.random((32, 32, 32, 64), seed=3, eps=0.25)
         cache = os.environ.get("QCU_TUNE_CACHE", "")
         have = bool(cache) and os.path.exists(cache)
-        for n in (1, 12):
-            if not have:
-                q.tune_hop(gk, 1, 12, n)
-            ms = q.bench_hop(gk, precision=1, recon=12, nrhs=n, iters=args.steps)
-            log(f"kernels-12rhs: f32 r12 32^3x64 nrhs={n} {ms*1e3:.1f} us/launch")
-        if have:
-            key = f"hop:1:12:{q.half_volume(gk.L)}:12"
-            with open(cache) as f:
-                rows = [ln.split() for ln in f if ln.startswith(key + " ")]
-            sh = dict(zip(q.SHAPE_KEYS, map(int, rows[-1][1:12])))
-            if sh["rb"] * sh["rpw"] * sh["rpt"] < 12:
-                sh["group_major"] ^= 1
-                q.set_hop_shape(gk, 1, 12, 12, sh)
-                ms = q.bench_hop(gk, precision=1, recon=12, nrhs=12, iters=args.steps)
-                log(f"kernels-12rhs: f32 r12 32^3x64 nrhs=12 other group order {sh} {ms*1e3:.1f} us/launch")
+        for prec in (1, 2):
+            for n in (1, 12):
+                if not have:
+                    q.tune_hop(gk, prec, 12, n)
+                ms = q.bench_hop(gk, precision=prec, recon=12, nrhs=n, iters=args.steps)
+                log(f"kernels-12rhs: {('f64', 'f32', 'f16')[prec]} r12 32^3x64 nrhs={n} {ms*1e3:.1f} us/launch")
         return 0
     out = {"metric": METRIC, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -562,12 +674,9 @@ def main():
     out["config"] = {"workload": "even-odd Wilson hop D_eo/D_oe, FP64, 18-real links, 16^3x32, antiperiodic t",
                      "lattice": list(L_MAIN), "half_volume": vh, "bytes_per_site": 2880,
                      "flop_per_site": 1320}
-    traffic = measured_traffic("hop_f64_r18_16x32")
-    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                       "kernel": "hop_kernel<F64,18,EPI_PLAIN>", "avg_launch_us": round(ms_event * 1e3, 3),
-                       "algorithmic_bytes_per_launch": bps}
-    log(f"hop f64 r18 16^3x32: {ms_event*1e3:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
+    out["roofline"] = roofline("hop_kernel<F64,18,EPI_PLAIN> 16^3x32", "f64", 18, vh, ms_event, "hop_f64_r18_16x32")
+    log(f"hop f64 r18 16^3x32: {ms_event*1e3:.2f} us/launch, {achieved:.0f} GB/s algorithmic, "
+        f"{out['roofline']['achieved']:.0f} GB/s compulsory")
     # the same hop as the median of 100 windows of 20 back-to-back launches, each
     # window timed by its own pair of HIP events
     win = sorted(q.bench_hop(g, precision=0, recon=18, nrhs=1, iters=20) for _ in range(100))
@@ -586,27 +695,7 @@ def main():
     out["dropin_16x32"] = dropin_section(q, g)
 
     # ---- CG: FP64 vs mixed on 16^3x32 Wilson ----------------------------------------
-    b = q.source_gaussian(L_MAIN, seed=2)
-    op = q.Operator(0.124, 0.0, -1, 12)
-    cg = {}
-    for name, prec in (("fp64", 0), ("mixed_fp32", 1), ("mixed_fp16", 2)):
-        sp = q.default_solve_params(tol=1e-10, precision=prec)
-        if prec == 0:
-            op64 = q.Operator(0.124, 0.0, -1, 18)
-            q.solve(g, op64, b, sp)  # warm (graph capture paths, buffers)
-            x, st = q.solve(g, op64, b, sp)
-        else:
-            q.solve(g, op, b, sp)
-            x, st = q.solve(g, op, b, sp)
-        cg[name] = {"seconds": round(st.seconds, 5), "iterations": st.iterations,
-                    "reliable_updates": st.reliable_updates, "true_residual": st.true_residual,
-                    "converged": bool(st.converged)}
-        log(f"cg {name}: {st.seconds*1e3:.2f} ms, {st.iterations} it, true res {st.true_residual:.2e}")
-    cg["speedup_mixed_fp32_vs_fp64"] = round(cg["fp64"]["seconds"] / cg["mixed_fp32"]["seconds"], 3)
-    cg["speedup_mixed_fp16_vs_fp64"] = round(cg["fp64"]["seconds"] / cg["mixed_fp16"]["seconds"], 3)
-    cg["problem"] = "16^3x32 Wilson, kappa=0.124, eps=0.2, antiperiodic t, 1 Gaussian source, tol 1e-10"
-    out["cg"] = cg
-    del np
+    out["cg"] = config2_section(q, g)
 
     # ---- larger lattices (beyond the Infinity Cache) ---------------------------------
     if not args.quick:
@@ -630,6 +719,9 @@ def main():
                                     "real_pct_hbm_peak": round(100 * real / HBM_PEAK_GBS, 1)})
                     large[key][f"{prec}_r{recon}"] = row
                     log(f"hop {prec} r{recon} {key}: {ms*1e3:.1f} us, {gbs:.0f} GB/s")
+                    if key == "32x32x32x64" and prec == "f64" and recon == 18:
+                        out["roofline_32x64"] = roofline("hop_kernel<F64,18,EPI_PLAIN> 32^3x64", prec, recon, vhl,
+                                                         ms, f"hop_{prec}_r{recon}_{key}")
             gl.destroy()
         out["dslash_large"] = large
         out["config1_wilson_8x8x8x8"] = config1_gpu(q)
