@@ -247,7 +247,8 @@ int qcuServerDestroy(qcuServer s);
  * links, clover and x (0 streaming, 1 cached), and the kernel (0: the above; 1: every
  * right-hand side in one block of 1, 2 or 4 waves per 64-site chunk, whose links are
  * loaded once into LDS -- wave groups, right-hand sides per wave and per thread and
- * group order then do not apply).
+ * group order then do not apply; 2: as 1, and a clover field shared by every
+ * right-hand side is staged in LDS too).
  * A tile that does not divide the lattice falls back to storage order.  QCU_TUNE=1 makes solves tune on first use;
  * QCU_TUNE_CACHE=<file> persists results across processes. */
 int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* shape, double* ms);

@@ -229,7 +229,7 @@ Tune tuned_hop(int prec, int recon, long vh, int nrhs, int epi) {
     // staged in LDS once (hop_lds_kernel; on 32^3x64 with 12 right-hand sides 22-29 %
     // faster than the best shape sharing link loads within a wave), 4 waves per block,
     // blocks round-robin over the XCDs
-    t.lds = 1;
+    t.lds = 2;  // a clover shared by every right-hand side staged in LDS as well
     t.block = nrhs >= 4 ? 256 : 128;
     t.xcd = 0;
     t.tz = -1;  // tiles of all of z by 8 t-planes (ignored where they do not fit)

@@ -149,7 +149,8 @@ struct HopArgs {
   int clov_shared;                       // every right-hand side uses clov[0]
   int nt_store;                          // nontemporal stores of out and loads of x (set by the launcher)
   Walk walk;                             // site traversal order
-  int lds;                               // 1: hop_lds_kernel (every right-hand side per block, links in LDS)
+  int lds;                               // 1: hop_lds_kernel (every right-hand side per block, links in LDS),
+                                         // 2: and a shared clover in LDS
 };
 
 // walk with (tile_z, tile_t) tiles, or storage order when the tiles do not divide the

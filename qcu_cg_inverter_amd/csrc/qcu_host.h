@@ -111,7 +111,8 @@ struct Tune {
   int rpw = 1;
   int spt = 1;
   int cached = 0;
-  int lds = 0;  // 1: every right-hand side in one block per 64-site chunk, its links staged in LDS
+  int lds = 0;  // 1: every right-hand side in one block per 64-site chunk, its links staged in LDS;
+                // 2: a clover shared by every right-hand side staged too
 };
 
 struct Context {

@@ -213,6 +213,14 @@ Tune autotune_hop(GaugeField& g, int prec, int recon, int nrhs, double* best_ms)
     epi_shapes.push_back(t);
   };
   for (size_t k = 0; k < top; ++k) add_shape(timed[k].second);
+  // the LDS kernel's shapes also with the clover staged in LDS (lds = 2; the clover
+  // epilogues use it when every right-hand side shares one clover field)
+  for (size_t k = 0; k < top; ++k)
+    if (timed[k].second.lds == 1) {
+      Tune t = timed[k].second;
+      t.lds = 2;
+      add_shape(t);
+    }
   int one_group = 0;
   for (size_t k = 0; k < timed.size() && one_group < 2; ++k) {
     Tune t = timed[k].second;
@@ -307,7 +315,7 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     t.spt = shape[9];
     t.cached = shape[10];
     t.lds = shape[11];
-    QREQ(t.lds == 0 || t.lds == 1, "lds must be 0 or 1");
+    QREQ(t.lds >= 0 && t.lds <= 2, "lds must be 0, 1 or 2");
     QREQ(!t.lds || t.block == 64 || t.block == 128 || t.block == 256, "the LDS kernel takes 1, 2 or 4 waves per block");
     QREQ(t.block == 64 || t.block == 128 || t.block == 192 || t.block == 256, "block must be 64, 128, 192 or 256");
     QREQ(t.rpw == 1 || t.rpw == 2 || t.rpw == 4, "right-hand sides per wave must be 1, 2 or 4");

@@ -419,11 +419,13 @@ def test_tuned_shape_keeps_bits(q, prec):
 
 @pytest.mark.parametrize("shape", [None, (256, 4, 3, 1, 0, 0, 0), (128, 2, 2, 1, 0, 0, 1),
                                    (192, 3, 1, 1, 0, 0, 0, 0, 4, 2, 1), (64, 1, 1, 0, 0, 0, 1, 0, 2, 4, 0),
-                                   (256, 1, 1, 1, 4, 4, 0, 0, 1, 1, 1, 1), (128, 1, 1, 0, 0, 0, 0, 0, 1, 2, 0, 1)])
+                                   (256, 1, 1, 1, 4, 4, 0, 0, 1, 1, 1, 1), (128, 1, 1, 0, 0, 0, 0, 0, 1, 2, 0, 1),
+                                   (256, 1, 1, 0, 8, 8, 0, 0, 1, 1, 1, 2), (64, 1, 1, 1, 0, 0, 0, 0, 1, 3, 0, 2)])
 def test_batch12_clover_mixed(q, shape):
     """12 right-hand sides in one batched FP32-inner clover solve (default launch
-    shape, or several right-hand sides per thread in either group order): every one
-    converges to 1e-10 and equals its own solve alone bit for bit."""
+    shape, several right-hand sides per thread in either group order, the LDS kernel
+    with links or links and clover staged): every one converges to 1e-10 and equals
+    its own solve alone bit for bit."""
     L = (8, 8, 8, 8)
     g = q.Gauge.random(L, seed=33, eps=0.25)
     if shape is not None:
