@@ -397,7 +397,7 @@ def config4_section(q):
     return out
 
 
-def dropin_section(q, g):
+def dropin_section(q, g=None):
     """The reference's entry points as a caller sees them, on the 16^3x32 gauge of the
     headline: host-synchronous dslashQcu (one parity, s = +1, periodic, kappa = 1) and
     fullDslashQcu (A - D with kappa = c_sw = 1, even-first full spinors), each call
@@ -406,6 +406,8 @@ def dropin_section(q, g):
 
     import numpy as np
 
+    if g is None:
+        g = q.Gauge.random(L_MAIN, seed=1, eps=0.2)
     L = g.L
     vh = q.half_volume(L)
     gd = q.DeviceBuffer.from_numpy(g.to_numpy())
@@ -597,7 +599,8 @@ def main():
     ap.add_argument("--kernels-12rhs", action="store_true",
                     help="profiler driver: copy kernel, then the 32^3x64 FP32 and FP16 hops with 1 and 12 "
                          "right-hand sides")
-    ap.add_argument("--section", choices=["config1", "config2", "config3", "config4", "serving", "dslash_12rhs"],
+    ap.add_argument("--section", choices=["config1", "config2", "config3", "config4", "serving", "dslash_12rhs",
+                                          "dropin"],
                     help="profiler driver: run one section only and print its JSON")
     ap.add_argument("--kernels-large", action="store_true",
                     help="profiler driver: 1 GiB copy, then the FP64 recon-18 and recon-12 hops on 32^3x64")
@@ -633,7 +636,7 @@ def main():
         return 0
     if args.section:
         fn = {"config1": config1_gpu, "config2": config2_section, "config3": config3_section,
-              "config4": config4_section,
+              "config4": config4_section, "dropin": dropin_section,
               "serving": lambda q_: serving_section(q_, args.serving_requests),
               "dslash_12rhs": multi_rhs_section}[args.section]
         print(json.dumps({args.section: fn(q)}))

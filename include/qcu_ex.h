@@ -180,6 +180,12 @@ int qcuSolveBatch(qcuGauge g, const qcuOperator* op, const qcuSolveParams* sp, i
 int qcuBenchHop(qcuGauge g, int precision, int recon, int nrhs, int iters, double* ms);
 /* device copy bandwidth: ms per copy of `bytes` (read + write) */
 int qcuBenchCopy(size_t bytes, int iters, double* ms);
+/* The link product of a 12-right-hand-side FP64 hop leg, Y = U H (a site's link as a
+ * 6x6 real matrix times its 24 projected colour vectors as 6x24), over site_legs
+ * site-legs (a multiple of 262144; a 32^3x64 launch has 8 per site, 8388608) with
+ * operands in registers: mfma = 0 on VALU FMAs, 1 on v_mfma_f64_16x16x4f64.
+ * ms = average per run; checksum = sum of every entry of Y (the same products both ways). */
+int qcuBenchLinkProduct(int mfma, long site_legs, int iters, double* ms, double* checksum);
 
 /* ---- serving ------------------------------------------------------------------ */
 /* A request loop with continuous batching.  A worker thread keeps one batch of

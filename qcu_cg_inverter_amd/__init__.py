@@ -139,6 +139,7 @@ PROTOTYPES = {
                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(SolveStats)]),
     "qcuBenchHop": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dbl)]),
     "qcuBenchCopy": (_int, [_sz, _int, ctypes.POINTER(_dbl)]),
+    "qcuBenchLinkProduct": (_int, [_int, ctypes.c_long, _int, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
     "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, ctypes.POINTER(_dbl)]),
     "qcuSetHopShape": (_int, [_vp, _int, _int, _int, _i4]),
     "qcuDefaultServerConfig": (None, [ctypes.POINTER(ServerConfig)]),
@@ -470,6 +471,13 @@ def set_hop_shape(g: Gauge, precision, recon, nrhs, shape: dict):
     """Pin the hopping kernel's launch shape for (precision, recon, nrhs) on g's volume."""
     v = (ctypes.c_int * len(SHAPE_KEYS))(*[int(shape.get(k, 1 if k in ("rpw", "spt") else 0)) for k in SHAPE_KEYS])
     check(lib().qcuSetHopShape(g.handle, precision, recon, nrhs, v))
+
+
+def bench_link_product(mfma: int, site_legs=8 * 1048576, iters=10):
+    """(ms, checksum) of the 12-RHS FP64 link product on VALU (0) or MFMA (1)."""
+    ms, cs = ctypes.c_double(), ctypes.c_double()
+    check(lib().qcuBenchLinkProduct(int(mfma), int(site_legs), iters, ctypes.byref(ms), ctypes.byref(cs)))
+    return ms.value, cs.value
 
 
 def bench_copy(nbytes, iters=20) -> float:

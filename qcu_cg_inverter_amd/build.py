@@ -26,6 +26,7 @@ SOURCES = [
     "k_blas.hip",
     "k_clover.hip",
     "k_gen.hip",
+    "k_linkprod.hip",
     "api.cpp",
     "solver.cpp",
     "cg_batch.cpp",

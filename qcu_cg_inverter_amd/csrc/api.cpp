@@ -831,6 +831,37 @@ int qcuBenchHop(qcuGauge h, int precision, int recon, int nrhs, int iters, doubl
   });
 }
 
+int qcuBenchLinkProduct(int mfma, long site_legs, int iters, double* ms, double* checksum) {
+  return guarded([&] {
+    ensure_init();
+    QREQ(ms && iters > 0, "qcuBenchLinkProduct: bad argument");
+    QREQ(mfma == 0 || mfma == 1, "mfma must be 0 or 1");
+    QREQ(site_legs > 0 && site_legs % 262144 == 0, "site_legs must be a positive multiple of 262144");
+    DevBuf* o = ctx().buf("linkprod", sizeof(double) * 262144);
+    launch_linkprod(mfma, site_legs, static_cast<double*>(o->p), stream());
+    hipEvent_t e0, e1;
+    QHIP(hipEventCreate(&e0));
+    QHIP(hipEventCreate(&e1));
+    QHIP(hipEventRecord(e0, stream()));
+    for (int i = 0; i < iters; ++i) launch_linkprod(mfma, site_legs, static_cast<double*>(o->p), stream());
+    QHIP(hipEventRecord(e1, stream()));
+    QHIP(hipEventSynchronize(e1));
+    QHIP(hipGetLastError());
+    float t = 0.f;
+    QHIP(hipEventElapsedTime(&t, e0, e1));
+    QHIP(hipEventDestroy(e0));
+    QHIP(hipEventDestroy(e1));
+    *ms = (double)t / iters;
+    if (checksum) {
+      std::vector<double> h(262144);
+      QHIP(hipMemcpy(h.data(), o->p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+      double sum = 0.0;
+      for (double v : h) sum += v;
+      *checksum = sum;
+    }
+  });
+}
+
 int qcuBenchCopy(size_t bytes, int iters, double* ms) {
   return guarded([&] {
     ensure_init();

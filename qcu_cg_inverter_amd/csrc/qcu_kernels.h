@@ -89,6 +89,12 @@ void launch_clover_convert(int prec, const void* src, void* dst, long vh, hipStr
 // packed F64 -> reference layout complex128 [Vh][2][6][6] (one parity)
 void launch_clover_unpack(const void* packed, void* out, long vh, hipStream_t st);
 
+// ------------------------------------------------------------ measurement --
+// S site-legs (a multiple of 262144) of the 12-right-hand-side link product Y = U H
+// (6x6 by 6x24, FP64) on VALU (mfma = 0) or v_mfma_f64_16x16x4f64 (mfma = 1), operands
+// in registers; out receives 262144 partial sums of Y (k_linkprod.hip)
+void launch_linkprod(int mfma, long site_legs, double* out, hipStream_t st);
+
 // ------------------------------------------------------------ generators ---
 // U_mu(x) = SU3-project(exp(i eps H)) into the interface gauge layout
 void launch_gauge_random(void* aos, long vh, uint64_t seed, double eps, hipStream_t st);

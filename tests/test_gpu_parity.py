@@ -507,3 +507,22 @@ def test_composites_match_oracle(q, csw):
             for dag, w in ((0, w0), (1, w1)):
                 got = q.apply_composite(g, op, kind, v, dagger=dag)
                 assert rel(got, w) < tol, (kappa, kind, dag)
+
+
+def test_link_product_mfma_matches_valu(q):
+    """The link-product measurement computes the same 12-RHS FP64 products on MFMA
+    (v_mfma_f64_16x16x4f64, padded operands) as on VALU: the sums of every product
+    entry agree to rounding, so its MFMA timing is of real work."""
+    S = 262144 * 4
+    ms_v, cs_v = q.bench_link_product(0, site_legs=S, iters=2)
+    ms_m, cs_m = q.bench_link_product(1, site_legs=S, iters=2)
+    assert ms_v > 0 and ms_m > 0
+    # the closed form: sum over r, c, j of U[r][c] H[c][j] = sum_c (sum_r U[r][c]) (sum_j H[c][j])
+    b = 1e-3 * (np.arange(S) & 1023)
+    want = 0.0
+    for c in range(6):
+        su = sum(b + 0.01 * (6 * r + c) for r in range(6))
+        sh = sum(0.5 * b + 0.001 * (j + 1) + 0.1 * c for j in range(24))
+        want += float((su * sh).sum())
+    assert abs(cs_v - want) <= 1e-12 * want, (cs_v, want)
+    assert abs(cs_m - want) <= 1e-12 * want, (cs_m, want)
