@@ -151,6 +151,8 @@ struct HopArgs {
   Walk walk;                             // site traversal order
   int lds;                               // 1: hop_lds_kernel (every right-hand side per block, links in LDS),
                                          // 2: and a shared clover in LDS
+  int cgw, cgq, cgr;                     // XCD column groups (xcd_remap >= 2, set by the launcher): W chunks
+  long cgnp;                             // per XCD of each Q-chunk sub-plane, R ranges of the NP sub-planes
 };
 
 // walk with (tile_z, tile_t) tiles, or storage order when the tiles do not divide the

@@ -517,6 +517,22 @@ __device__ __forceinline__ void clover_mul(const T a[72], const cx<T> v[12], cx<
   for (int b = 0; b < 2; ++b) clover_mul_block(a + 36 * b, v + 6 * b, out + 6 * b);
 }
 
+// XCD column groups: the hardware hands workgroup b to XCD x = b mod 8.  Give that XCD
+// W consecutive 64-site chunks of every Q-chunk sub-plane of the walk, in one of
+// R = 8 W / Q ranges of the NP sub-planes, in walk order.  Each XCD then sweeps a
+// column W chunks wide: the sub-planes a site's t and z neighbours live in are a few
+// steps away on the same XCD, so its L2 holds them, while the column's edge rows are
+// fetched by two XCDs.  W = 1 is plain round-robin.  A bijection on [0, NP Q) when
+// Q % W == 0, 8 W % Q == 0 and NP % R == 0 (the launcher checks).
+__device__ __forceinline__ unsigned colgroup_block(unsigned b, int w, int q, int r, long np) {
+  const unsigned x = b & 7u, k = b >> 3;
+  const unsigned per = (unsigned)(q / w);  // XCDs per range
+  const unsigned rr = x / per, gx = x - rr * per;
+  const unsigned pl = k / (unsigned)w, i = k - pl * (unsigned)w;
+  const unsigned P = rr * (unsigned)(np / r) + pl;
+  return P * (unsigned)q + gx * (unsigned)w + i;
+}
+
 // XCD-aware block order: the hardware hands workgroup b to XCD (b mod 8); remap so
 // that each XCD walks one contiguous range of sites (its L2 then sees neighbours).
 __device__ __forceinline__ unsigned remap_block(unsigned b, unsigned nb, int on) {

@@ -93,11 +93,15 @@ static std::vector<Tune> candidates(const Geom& geo, int prec, int nrhs) {
   // tried: 784 us against 847 us for 4 x 4 tiles)
   std::vector<std::array<int, 3>> ltiles;
   for (auto& tl : tiles) ltiles.push_back({tl[0], tl[1], tl[2]});
+  ltiles.push_back({0, geo.L[2], 16});
   ltiles.push_back({0, geo.L[2], 8});
   ltiles.push_back({0, geo.L[2], 4});
+  // block orders: none (round-robin over the XCDs), contiguous ranges, and XCD column
+  // groups 2 and 4 chunks wide
+  const int xorders[] = {0, 1, 2, 4};
   if (nrhs > 1)
     for (auto& tl : ltiles)
-      for (int xcd = 0; xcd <= 1; ++xcd)
+      for (int xcd : xorders)
         for (int nw = 1; nw <= 4; nw *= 2) {
           if (nw > nrhs) continue;
           Tune t;
@@ -323,7 +327,8 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     QREQ(t.cached == 0 || t.cached == 1, "load policy must be 0 (streaming) or 1 (cached)");
     QREQ(t.rb >= 1 && t.block % (64 * t.rb) == 0, "rb must divide block into 64-thread groups");
     QREQ(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1), "rpt must be 1..4 (1 for FP64)");
-    QREQ((t.xcd == 0 || t.xcd == 1) && (t.gm == 0 || t.gm == 1), "xcd and group order are 0 or 1");
+    QREQ((t.xcd == 0 || t.xcd == 1 || (t.lds && (t.xcd == 2 || t.xcd == 4))) && (t.gm == 0 || t.gm == 1),
+         "xcd is 0 or 1 (2 or 4: XCD column groups, LDS kernel only); group order is 0 or 1");
     QREQ(t.tz >= 0 && t.tt >= 0 && t.ty >= 0 && (t.tz == 0) == (t.tt == 0),
          "tile extents must be non-negative, tz and tt both 0 or both positive");
     GaugeField& g = *reinterpret_cast<GaugeField*>(h);

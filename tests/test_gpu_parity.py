@@ -460,7 +460,10 @@ def test_hop_shapes_keep_bits(q, prec):
               (256, 4, 1, 1, 0, 0, 0, 0, 4, 1, 0), (256, 2, 1, 1, 8, 8, 1, 0, 2, 2, 1),
               # the LDS-staged kernel (last entry 1): 4, 2 and 1 waves per block
               (256, 1, 1, 1, 4, 4, 0, 8, 1, 1, 0, 1), (128, 1, 1, 0, 0, 0, 0, 0, 1, 2, 1, 1),
-              (64, 1, 1, 1, 8, 8, 0, 0, 1, 3, 0, 1), (256, 1, 1, 0, 2, 4, 0, 8, 1, 1, 1, 1)]
+              (64, 1, 1, 1, 8, 8, 0, 0, 1, 3, 0, 1), (256, 1, 1, 0, 2, 4, 0, 8, 1, 1, 1, 1),
+              # XCD column groups 2 and 4 chunks wide (4 falls back to round-robin here)
+              (256, 1, 1, 2, 8, 16, 0, 0, 1, 1, 1, 1), (128, 1, 1, 2, 0, 0, 0, 0, 1, 1, 0, 1),
+              (256, 1, 1, 4, 8, 8, 0, 0, 1, 1, 1, 2)]
     for sh in shapes:
         shape = dict(zip(q.SHAPE_KEYS, sh))
         if prec == 0:

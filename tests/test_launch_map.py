@@ -71,3 +71,49 @@ def test_launch_covers_each_pair_once(vh, nrhs, shape):
     block, rb, rpw, rpt, spt = shape
     for gm, xcd in itertools.product((0, 1), (0, 1)):
         launch(vh, nrhs, block, rb, rpw, rpt, gm, xcd, spt)
+
+
+def colgroup_block(b, w, q, r, np_):
+    """colgroup_block in qcu_fields.h: XCD x = b % 8 takes w consecutive chunks of every
+    q-chunk sub-plane in one of r ranges of the np sub-planes."""
+    x, k = b & 7, b >> 3
+    per = q // w
+    rr, gx = x // per, x % per
+    pl, i = k // w, k % w
+    P = rr * (np_ // r) + pl
+    return P * q + gx * w + i
+
+
+def colgroup_params(vh, sub, w, spt=1):
+    """The launcher's validity check (launch_hop_prec in k_hop.inc); None: round-robin."""
+    q = sub // 64
+    np_ = vh // sub if sub % 64 == 0 else 0
+    nsb = -(-vh // 64)
+    if sub % 64 or q % w or (8 * w) % q or np_ % ((8 * w) // q) or spt != 1 or np_ * q != nsb:
+        return None
+    return w, q, (8 * w) // q, np_
+
+
+@pytest.mark.parametrize("L", [(32, 32, 32, 64), (16, 16, 16, 32), (32, 16, 8, 16), (16, 16, 8, 16)])
+@pytest.mark.parametrize("w", [2, 4])
+def test_xcd_column_groups(L, w):
+    """The XCD column-group order of the LDS kernel is a permutation of the 64-site
+    chunks, and each XCD gets w consecutive chunks of every sub-plane of its range."""
+    lxh = L[0] // 2
+    vh = lxh * L[1] * L[2] * L[3]
+    sub = lxh * L[1]  # whole xy-planes (the tiles of all z by tt t-planes)
+    p = colgroup_params(vh, sub, w)
+    if p is None:
+        pytest.skip("column groups do not tile this walk; the launcher uses round-robin")
+    w_, q, r, np_ = p
+    nsb = vh // 64
+    img = [colgroup_block(b, w_, q, r, np_) for b in range(nsb)]
+    assert sorted(img) == list(range(nsb))
+    for x in range(8):
+        mine = sorted(img[b] for b in range(x, nsb, 8))
+        by_plane = {}
+        for c in mine:
+            by_plane.setdefault(c // q, []).append(c % q)
+        assert all(len(v) == w and v == list(range(v[0], v[0] + w)) for v in by_plane.values())
+        planes = sorted(by_plane)
+        assert planes == list(range(planes[0], planes[0] + len(planes)))  # one range of sub-planes
