@@ -318,28 +318,16 @@ static void require_aligned(const void* p, const char* what) {
   QREQ((reinterpret_cast<uintptr_t>(p) & 15) == 0, std::string(what) + " must be 16-byte aligned (complex128)");
 }
 
-// The packed copy and clover fields behind fullDslashQcu and cg_inverter.  The
-// reference reads the caller's gauge on every call (src/dslash.cu:73,
-// src/cg_inverter.cu:312-339), so a caller may change links in place between calls
-// (an HMC update, smearing) or pass a reused address.  The copy is therefore checked
-// against a fingerprint of the caller's buffer on every call -- one read of the gauge
-// -- and remade, with its clover fields and solver state dropped, when it differs.
-// QCU_DROPIN_GAUGE_CACHE=1 skips the check.  One copy is kept: a new pointer or
-// geometry replaces it.
-static GaugeField& dropin_gauge(void* gauge, const QcuParam* param, bool reload) {
-  Context& c = ctx();
-  QREQ(gauge != nullptr, "gauge pointer is NULL");
-  QREQ(param != nullptr, "QcuParam pointer is NULL");
-  require_aligned(gauge, "gauge");
-  validate_lattice(param->lattice_size);
-  Context::Dropin& d = c.dropin;
-  const bool same_geom = d.g && memcmp(d.g->L, param->lattice_size, sizeof(int) * 4) == 0;
-  const bool same = same_geom && d.ptr == gauge;
-  if (same && !reload && dropin_trust_pointer()) return *d.g;
-  const long vh = (long)param->lattice_size[0] * param->lattice_size[1] * param->lattice_size[2] *
-                  param->lattice_size[3] / 2;
-  const uint64_t fp = gauge_fingerprint(gauge, vh);
-  if (same && !reload && fp == d.fp) return *d.g;
+// pinned host word the fingerprint of a deferred check lands in
+static uint64_t* fingerprint_host() {
+  static uint64_t* h = nullptr;
+  if (!h) QHIP(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(uint64_t), hipHostMallocDefault));
+  return h;
+}
+
+// remake the drop-in copy from the caller's gauge, whose fingerprint is fp
+static void dropin_rebuild(void* gauge, const QcuParam* param, bool same_geom, uint64_t fp) {
+  Context::Dropin& d = ctx().dropin;
   if (d.g) {
     QHIP(hipStreamSynchronize(stream()));
     drop_cached_batches(d.g.get());
@@ -358,7 +346,54 @@ static GaugeField& dropin_gauge(void* gauge, const QcuParam* param, bool reload)
   QHIP(hipGetLastError());
   d.ptr = gauge;
   d.fp = fp;
+}
+
+// The packed copy and clover fields behind fullDslashQcu and cg_inverter.  The
+// reference reads the caller's gauge on every call (src/dslash.cu:73,
+// src/cg_inverter.cu:312-339), so a caller may change links in place between calls
+// (an HMC update, smearing) or pass a reused address.  The copy is therefore checked
+// against a fingerprint of the caller's buffer on every call -- one read of the gauge
+// -- and remade, with its clover fields and solver state dropped, when it differs.
+// QCU_DROPIN_GAUGE_CACHE=1 skips the check.  One copy is kept: a new pointer or
+// geometry replaces it.  With `deferred` (fullDslashQcu), a copy that exists for this
+// pointer and geometry is returned at once and the fingerprint is only queued: the
+// caller runs its work, synchronises, and asks dropin_gauge_changed whether to redo it
+// with a remade copy, which saves a synchronisation in the common case.
+static GaugeField& dropin_gauge(void* gauge, const QcuParam* param, bool reload, bool* deferred = nullptr) {
+  Context& c = ctx();
+  if (deferred) *deferred = false;
+  QREQ(gauge != nullptr, "gauge pointer is NULL");
+  QREQ(param != nullptr, "QcuParam pointer is NULL");
+  require_aligned(gauge, "gauge");
+  validate_lattice(param->lattice_size);
+  Context::Dropin& d = c.dropin;
+  const bool same_geom = d.g && memcmp(d.g->L, param->lattice_size, sizeof(int) * 4) == 0;
+  const bool same = same_geom && d.ptr == gauge;
+  if (same && !reload && dropin_trust_pointer()) return *d.g;
+  const long vh = (long)param->lattice_size[0] * param->lattice_size[1] * param->lattice_size[2] *
+                  param->lattice_size[3] / 2;
+  if (deferred && same && !reload) {
+    DevBuf* sbuf = ctx().buf("fingerprint", sizeof(uint64_t) * (kFingerprintBlocks + 1));
+    launch_fingerprint(gauge, (size_t)8 * vh * 9 * 16, static_cast<uint64_t*>(sbuf->p), stream());
+    QHIP(hipGetLastError());
+    QHIP(hipMemcpyAsync(fingerprint_host(), sbuf->p, sizeof(uint64_t), hipMemcpyDeviceToHost, stream()));
+    *deferred = true;
+    return *d.g;
+  }
+  const uint64_t fp = gauge_fingerprint(gauge, vh);
+  if (same && !reload && fp == d.fp) return *d.g;
+  dropin_rebuild(gauge, param, same_geom, fp);
   return *d.g;
+}
+
+// after a deferred dropin_gauge and a synchronisation of the library stream: true
+// when the caller's gauge had changed, in which case the copy has been remade and the
+// caller redoes its work
+static bool dropin_gauge_changed(void* gauge, const QcuParam* param) {
+  const uint64_t fp = *fingerprint_host();
+  if (fp == ctx().dropin.fp) return false;
+  dropin_rebuild(gauge, param, true, fp);
+  return true;
 }
 
 extern "C" void initGridSize(QcuGrid_t* grid, QcuParam* p_param, void* gauge, void* fermion_in, void* fermion_out) {
@@ -415,10 +450,15 @@ extern "C" void fullDslashQcu(void* fermion_out, void* fermion_in, void* gauge, 
     QREQ(fermion_out && fermion_in, "fullDslashQcu: NULL spinor");
     require_aligned(fermion_in, "fermion_in");
     require_aligned(fermion_out, "fermion_out");
-    GaugeField& g = dropin_gauge(gauge, param, false);
+    bool deferred = false;
+    GaugeField& g = dropin_gauge(gauge, param, false, &deferred);
     qcuOperator op = {1.0, 1.0, 1, 12};
     apply_m_iface(g, op, fermion_out, fermion_in, dagger_flag ? 1 : 0);
     QHIP(hipStreamSynchronize(stream()));
+    if (deferred && dropin_gauge_changed(gauge, param)) {  // the links changed in place: redo
+      apply_m_iface(*ctx().dropin.g, op, fermion_out, fermion_in, dagger_flag ? 1 : 0);
+      QHIP(hipStreamSynchronize(stream()));
+    }
   });
 }
 
