@@ -294,15 +294,18 @@ struct SpinorIO<F16> {
     float m = 0.f;
 #pragma unroll
     for (int c = 0; c < 12; ++c) m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
-    float inv = m > 0.f ? 1.0f / m : 0.f;
+    // 1 / m overflows float for m below 2^-128 (denormal norms): such a site is scaled
+    // by 2^64 first, exactly, so its halves stay in [-1, 1]; other sites see s = 1
+    const float s = m < 0x1p-100f ? 0x1p64f : 1.0f;
+    const float inv = m > 0.f ? 1.0f / (m * s) : 0.f;
     uint4* b = site_block<3>(static_cast<uint4*>(base), i);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float f[8];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        f[2 * j] = v[4 * k + j].re * inv;
-        f[2 * j + 1] = v[4 * k + j].im * inv;
+        f[2 * j] = (v[4 * k + j].re * s) * inv;
+        f[2 * j + 1] = (v[4 * k + j].im * s) * inv;
       }
       stp<NTS>(b + k * kSiteBlock, pack8(f));
     }
