@@ -8,22 +8,30 @@
 // one pass with no conversion kernels and no copy of the gauge, so a gauge the caller
 // changed in place is always the one applied.
 //
-// A thread owns one output site; the arithmetic is hop_dir of k_hop.inc, in the same
-// leg order, so the result is bit-identical to the SoA hop with reconstruct-12.  The
-// records a wave needs for one leg (64 neighbour spinors of 192 B, or 64 links of
+// A block of four waves owns 64 output sites; wave mu computes the two legs of
+// direction mu (forward, then backward) for all 64, with hop_dir's arithmetic
+// (hop_dir_t), and leaves their link products in LDS; then wave r adds spin row r of
+// every site up over the eight legs in hop_dir's order, so the result is bit-identical
+// to the SoA hop with reconstruct-12.  One call of 16^3x32 is only 1,024 such 64-site
+// chunks -- one wave per SIMD if a wave did all eight legs -- so splitting the legs
+// over four waves is what lets the memory latency of one wave overlap another's.
+//
+// The records a wave needs for one leg (64 neighbour spinors of 192 B and 64 links of
 // which 96 B are read) are gathered as consecutive 16-byte words -- lane l loads word
-// f = 64 k + l of the list, record f / W, word f % W -- so each load instruction
-// reads a contiguous run wherever the neighbours of consecutive sites are consecutive
-// (all but the lattice-edge rows).  The words go through the wave's LDS tile, padded
-// to an odd number of 16-byte words per record so that lanes reading their own record
-// hit distinct banks.  The output goes back the same way: each lane writes its 12
-// words to the tile and the wave stores 768 consecutive words.
+// f = 64 k + l of the list, record f / W, word f % W -- so each load instruction reads
+// a contiguous run wherever the neighbours of consecutive sites are consecutive (all
+// but the lattice-edge rows).  All 18 loads of a leg are issued before the words go
+// into the wave's LDS tiles, padded to an odd number of 16-byte words per record so
+// that lanes reading their own record hit distinct banks.  The output leaves through
+// LDS the same way: the block stores 768 consecutive words.
 #include "k_hop.inc"
 
 namespace qcu {
 
 constexpr int kIfaceSpPad = 13;  // LDS words per spinor record (12 + 1)
 constexpr int kIfaceLkPad = 7;   // LDS words per link record (6 + 1)
+// one wave's LDS region: its spinor and link tiles (20 KB), later its legs' link products
+constexpr int kIfaceWave = 64 * (kIfaceSpPad + kIfaceLkPad);
 
 // a wave's LDS accesses are processed in order; this keeps the compiler from moving
 // the wave's own loads and stores across the point where lanes trade records
@@ -33,98 +41,145 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// keeps the compiler from computing a lane's tile offsets once and holding them in
-// registers across all eight legs (they are cheaper to recompute than to keep)
-__device__ __forceinline__ int fresh(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
+// the value v of lane j
+__device__ __forceinline__ int from_lane(int v, int j) { return __builtin_amdgcn_ds_bpermute(j << 2, v); }
 
-// leg L = 2 mu + (backward): neighbour spinor records I[L][.], link records J[L][.]
+// link products t of leg (MU, FWD) for the wave's 64 sites: lane l's neighbour spinor
+// is record `sp` of `in`, its link record `lk` of the gauge's (mu, link parity) field
 template <int MU, bool FWD, int DAG>
-__device__ __forceinline__ void iface_leg(const double2* __restrict__ in, const double2* __restrict__ gauge,
-                                          long vh, int parity, double bs, double2* T, const int* I, const int* J,
-                                          int lane_in, cx<double> acc[12]) {
-  const int lane = fresh(lane_in);
+__device__ __forceinline__ void iface_leg_t(const double2* __restrict__ in, const double2* __restrict__ gauge,
+                                            long vh, int parity, double bs, int sp, int lk, double2* W, int lane,
+                                            cx<double> t[2][3]) {
   const int lpar = FWD ? parity : 1 - parity;
+  const double2* gb = gauge + (long)(MU * 2 + lpar) * vh * 9;
+  double2* TS = W;
+  double2* TL = W + 64 * kIfaceSpPad;
+  double2 sw[12], lw[6];
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
     const int f = k * 64 + lane, j = f / 12, w = f - 12 * j;
-    T[j * kIfaceSpPad + w] = in[(long)I[j] * 12 + w];
+    sw[k] = in[(long)from_lane(sp, j) * 12 + w];
   }
-  wave_sync();
-  cx<double> psi[12];
-#pragma unroll
-  for (int w = 0; w < 12; ++w) {
-    const double2 q = T[lane * kIfaceSpPad + w];
-    psi[w] = {q.x, q.y};
-  }
-  wave_sync();
-  const double2* gb = gauge + (long)(MU * 2 + lpar) * vh * 9;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int f = k * 64 + lane, j = f / 6, w = f - 6 * j;
-    T[j * kIfaceLkPad + w] = gb[(long)J[j] * 9 + w];
+    lw[k] = gb[(long)from_lane(lk, j) * 9 + w];
+  }
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const int f = k * 64 + lane, j = f / 12, w = f - 12 * j;
+    TS[j * kIfaceSpPad + w] = sw[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int f = k * 64 + lane, j = f / 6, w = f - 6 * j;
+    TL[j * kIfaceLkPad + w] = lw[k];
   }
   wave_sync();
-  cx<double> u[9];
+  cx<double> psi[12], u[9];
+#pragma unroll
+  for (int w = 0; w < 12; ++w) {
+    const double2 q = TS[lane * kIfaceSpPad + w];
+    psi[w] = {q.x, q.y};
+  }
 #pragma unroll
   for (int e = 0; e < 6; ++e) {
-    const double2 q = T[lane * kIfaceLkPad + e];
+    const double2 q = TL[lane * kIfaceLkPad + e];
     u[e] = {q.x, q.y};
   }
+  wave_sync();  // the tiles are free again
   recon_row2(u);
-  wave_sync();  // the next leg overwrites T
-  hop_dir<MU, FWD, DAG>(acc, u, psi, bs);
+  hop_dir_t<MU, FWD, DAG>(u, psi, bs, t);
 }
 
-// one wave per block; wave b covers half sites [64 b, 64 b + 64)
+// wave MU: both legs of direction MU; their products go to the wave's region as
+// [leg (0 forward, 1 backward)][6 = 3 r + i][64 lanes]
+template <int MU, int DAG>
+__device__ __forceinline__ void iface_wave(const double2* __restrict__ in, const double2* __restrict__ gauge,
+                                           const Geom& g, int parity, double tbc, long site, int lane, double2* W) {
+  int c[4];
+  SiteNb n;
+  decode_site(g, site, parity, c, n);
+  const double bf = (MU == 3 && n.tf) ? tbc : 1.0, bb = (MU == 3 && n.tb) ? tbc : 1.0;
+  cx<double> tf[2][3], tb[2][3];
+  iface_leg_t<MU, true, DAG>(in, gauge, g.vh, parity, bf, (int)n.f[MU], (int)site, W, lane, tf);
+  iface_leg_t<MU, false, DAG>(in, gauge, g.vh, parity, bb, (int)n.b[MU], (int)n.b[MU], W, lane, tb);
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      W[(3 * r + i) * 64 + lane] = double2{tf[r][i].re, tf[r][i].im};
+      W[(6 + 3 * r + i) * 64 + lane] = double2{tb[r][i].re, tb[r][i].im};
+    }
+}
+
+// o += the spin-row-R terms of leg L (direction L / 2, forward when L is even), whose
+// products wave L / 2 left in S
+template <int L, int DAG, int R>
+__device__ __forceinline__ void iface_add_leg(const double2* S, int lane, cx<double> o[3]) {
+  constexpr int MU = L / 2;
+  constexpr bool FWD = (L & 1) == 0;
+  constexpr int src = R < 2 ? R : GCOL[MU][R];
+  const double2* P = S + MU * kIfaceWave + (L & 1) * 6 * 64;
+  cx<double> t[2][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double2 q = P[(3 * src + i) * 64 + lane];
+    t[src][i] = {q.x, q.y};
+    t[1 - src][i] = {0.0, 0.0};
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = o[i] + leg_row<MU, FWD, DAG, R>(t, i);
+}
+
+template <int DAG, int R>
+__device__ __forceinline__ void iface_row(const double2* S, int lane, cx<double> o[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = {0.0, 0.0};
+  iface_add_leg<0, DAG, R>(S, lane, o);
+  iface_add_leg<1, DAG, R>(S, lane, o);
+  iface_add_leg<2, DAG, R>(S, lane, o);
+  iface_add_leg<3, DAG, R>(S, lane, o);
+  iface_add_leg<4, DAG, R>(S, lane, o);
+  iface_add_leg<5, DAG, R>(S, lane, o);
+  iface_add_leg<6, DAG, R>(S, lane, o);
+  iface_add_leg<7, DAG, R>(S, lane, o);
+}
+
+// block b covers half sites [64 b, 64 b + 64); 256 threads, wave w = direction w, then
+// spin row w
 template <int DAG>
-__global__ void __launch_bounds__(64) hop_iface_kernel(const double2* __restrict__ in, double2* __restrict__ out,
-                                                       const double2* __restrict__ gauge, const Geom g, int parity,
-                                                       double tbc, double coef) {
-  __shared__ double2 T[64 * kIfaceSpPad];
-  __shared__ int I[8][64], J[8][64];
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(256, 2) hop_iface_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                          const double2* __restrict__ gauge, const Geom g,
+                                                          int parity, double tbc, double coef) {
+  __shared__ double2 S[4 * kIfaceWave];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long w0 = (long)blockIdx.x * 64;
   const int nlive = (int)(g.vh - w0 < 64 ? g.vh - w0 : 64);  // grid = ceil(vh / 64): nlive >= 1
   const long site = w0 + (lane < nlive ? lane : nlive - 1);   // past the end: a valid site, not stored
-  double bs_tf, bs_tb;
-  {
-    int c[4];
-    SiteNb n;
-    decode_site(g, site, parity, c, n);
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-      I[2 * mu][lane] = (int)n.f[mu];
-      J[2 * mu][lane] = (int)site;  // U_mu(x)
-      I[2 * mu + 1][lane] = (int)n.b[mu];
-      J[2 * mu + 1][lane] = (int)n.b[mu];  // U_mu(x - mu)
-    }
-    bs_tf = n.tf ? tbc : 1.0;
-    bs_tb = n.tb ? tbc : 1.0;
+  double2* W = S + wv * kIfaceWave;
+  switch (wv) {
+    case 0: iface_wave<0, DAG>(in, gauge, g, parity, tbc, site, lane, W); break;
+    case 1: iface_wave<1, DAG>(in, gauge, g, parity, tbc, site, lane, W); break;
+    case 2: iface_wave<2, DAG>(in, gauge, g, parity, tbc, site, lane, W); break;
+    default: iface_wave<3, DAG>(in, gauge, g, parity, tbc, site, lane, W); break;
   }
-  wave_sync();
-  cx<double> acc[12];
+  __syncthreads();
+  cx<double> o[3];
+  switch (wv) {
+    case 0: iface_row<DAG, 0>(S, lane, o); break;
+    case 1: iface_row<DAG, 1>(S, lane, o); break;
+    case 2: iface_row<DAG, 2>(S, lane, o); break;
+    default: iface_row<DAG, 3>(S, lane, o); break;
+  }
+  __syncthreads();  // every wave has read the products; S takes the output tile
 #pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = {0.0, 0.0};
-  const double one = 1.0;
-  iface_leg<0, true, DAG>(in, gauge, g.vh, parity, one, T, I[0], J[0], lane, acc);
-  iface_leg<0, false, DAG>(in, gauge, g.vh, parity, one, T, I[1], J[1], lane, acc);
-  iface_leg<1, true, DAG>(in, gauge, g.vh, parity, one, T, I[2], J[2], lane, acc);
-  iface_leg<1, false, DAG>(in, gauge, g.vh, parity, one, T, I[3], J[3], lane, acc);
-  iface_leg<2, true, DAG>(in, gauge, g.vh, parity, one, T, I[4], J[4], lane, acc);
-  iface_leg<2, false, DAG>(in, gauge, g.vh, parity, one, T, I[5], J[5], lane, acc);
-  iface_leg<3, true, DAG>(in, gauge, g.vh, parity, bs_tf, T, I[6], J[6], lane, acc);
-  iface_leg<3, false, DAG>(in, gauge, g.vh, parity, bs_tb, T, I[7], J[7], lane, acc);
-  const int ln = fresh(lane);
+  for (int i = 0; i < 3; ++i) S[lane * kIfaceSpPad + 3 * wv + i] = double2{coef * o[i].re, coef * o[i].im};
+  __syncthreads();
 #pragma unroll
-  for (int w = 0; w < 12; ++w) T[ln * kIfaceSpPad + w] = double2{coef * acc[w].re, coef * acc[w].im};
-  wave_sync();
-#pragma unroll
-  for (int k = 0; k < 12; ++k) {
-    const int f = k * 64 + ln, j = f / 12, w = f - 12 * j;
-    if (j < nlive) out[(w0 + j) * 12 + w] = T[j * kIfaceSpPad + w];
+  for (int k = 0; k < 3; ++k) {
+    const int f = k * 256 + (int)threadIdx.x, j = f / 12, w = f - 12 * j;
+    if (j < nlive) out[(w0 + j) * 12 + w] = S[j * kIfaceSpPad + w];
   }
 }
 
@@ -134,8 +189,8 @@ void launch_hop_iface(const void* in, void* out, const void* gauge_aos, const Ge
   const double2* i = static_cast<const double2*>(in);
   double2* o = static_cast<double2*>(out);
   const double2* u = static_cast<const double2*>(gauge_aos);
-  if (dagger) hipLaunchKernelGGL(hop_iface_kernel<1>, grid, dim3(64), 0, st, i, o, u, g, parity, tbc, coef);
-  else hipLaunchKernelGGL(hop_iface_kernel<0>, grid, dim3(64), 0, st, i, o, u, g, parity, tbc, coef);
+  if (dagger) hipLaunchKernelGGL(hop_iface_kernel<1>, grid, dim3(256), 0, st, i, o, u, g, parity, tbc, coef);
+  else hipLaunchKernelGGL(hop_iface_kernel<0>, grid, dim3(256), 0, st, i, o, u, g, parity, tbc, coef);
 }
 
 }  // namespace qcu
