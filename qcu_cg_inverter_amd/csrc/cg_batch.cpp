@@ -9,7 +9,7 @@
 namespace qcu {
 
 size_t CgBatch::bytes_for(long vh, int prec, int cap) {
-  size_t fb = spinor_bytes(F64, vh), pb = spinor_bytes(prec, vh), xb = spinor_bytes(F32, vh);
+  size_t fb = spinor_bytes(F64, vh), pb = spinor_bytes(prec, vh), xb = spinor_bytes(iterate_precision(prec), vh);
   size_t s = 4 * fb * cap + (prec == F64 ? 4 : 5) * pb * cap + (prec == F64 ? 0 : xb * cap);
   s += sizeof(double) * (4 * n_partials(vh) * cap + cap) + sizeof(CgScal) * cap + 2 * (size_t)vh * 192;
   return s;
@@ -17,7 +17,7 @@ size_t CgBatch::bytes_for(long vh, int prec, int cap) {
 
 CgBatch::CgBatch(GaugeField& g, int prec, int recon, int t_boundary, bool clover, int capacity,
                  const qcuSolveParams& sp, hipStream_t st)
-    : g_(g), prec_(prec), xprec_(prec == F64 ? F64 : F32), recon_(recon), tbc_(t_boundary), clover_(clover),
+    : g_(g), prec_(prec), xprec_(iterate_precision(prec)), recon_(recon), tbc_(t_boundary), clover_(clover),
       cap_(capacity), sp_(sp), st_(st) {
   QREQ(capacity >= 1 && capacity <= kMaxRhs, "batch capacity must be 1..16");
   QREQ(prec >= 0 && prec <= 2, "bad precision");

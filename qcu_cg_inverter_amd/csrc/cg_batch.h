@@ -28,6 +28,15 @@ struct SlotResult {
   qcuSolveStats stats;
 };
 
+// Precision of the inner iterate x between reliable updates: FP64 for the plain
+// solve, else FP32.  x gathers the corrections of one reliable-update cycle.  Kept in
+// FP16 next to an FP16 inner loop it saves a quarter of the p / x update's traffic but
+// its rounding (2^-11 of the site norm per update) cost iterations: 109-110 instead of
+// 107 on the 12-source 32^3x64 clover batch (1 % faster), 107 instead of 101 on the
+// 24^3x48 clover solve and 117 instead of 107 on the 16^3x32 Wilson solve (5-7 %
+// slower), measured on MI355X.
+inline int iterate_precision(int prec) { return prec == F64 ? F64 : F32; }
+
 class CgBatch {
  public:
   // prec: inner precision; clover: slots carry a clover term (csw != 0)
