@@ -261,6 +261,10 @@ int qcuTuneHop(qcuGauge g, int precision, int recon, int nrhs, int* shape, doubl
 /* Sets the launch shape (the 12 ints above, for every epilogue) used for (precision, recon, nrhs) on this
  * gauge's volume, e.g. one tuned earlier.  Every valid shape gives the same bits. */
 int qcuSetHopShape(qcuGauge g, int precision, int recon, int nrhs, const int* shape);
+/* The launch shape (the 12 ints above) the plain hop uses for (precision, recon, nrhs) on this gauge's volume:
+ * one tuned, pinned or read from QCU_TUNE_CACHE (whose lines with a shape qcuSetHopShape would refuse are
+ * skipped), else the default.  A tile extent of -1 stands for the whole lattice in that direction. */
+int qcuGetHopShape(qcuGauge g, int precision, int recon, int nrhs, int* shape);
 
 #ifdef __cplusplus
 }

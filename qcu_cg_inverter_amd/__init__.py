@@ -142,6 +142,7 @@ PROTOTYPES = {
     "qcuBenchLinkProduct": (_int, [_int, ctypes.c_long, _int, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
     "qcuTuneHop": (_int, [_vp, _int, _int, _int, _i4, ctypes.POINTER(_dbl)]),
     "qcuSetHopShape": (_int, [_vp, _int, _int, _int, _i4]),
+    "qcuGetHopShape": (_int, [_vp, _int, _int, _int, _i4]),
     "qcuDefaultServerConfig": (None, [ctypes.POINTER(ServerConfig)]),
     "qcuServerCreate": (_int, [ctypes.POINTER(_vp), ctypes.POINTER(ServerConfig)]),
     "qcuServerSubmit": (_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
@@ -465,6 +466,13 @@ def tune_hop(g: Gauge, precision=PREC_DOUBLE, recon=18, nrhs=1):
 
 SHAPE_KEYS = ("block", "rb", "rpt", "xcd", "tile_z", "tile_t", "group_major", "tile_y", "rpw", "spt", "cached",
               "lds")
+
+
+def get_hop_shape(g: Gauge, precision, recon, nrhs) -> dict:
+    """The launch shape the plain hop uses for (precision, recon, nrhs) on g's volume."""
+    v = (ctypes.c_int * len(SHAPE_KEYS))()
+    check(lib().qcuGetHopShape(g.handle, precision, recon, nrhs, v))
+    return dict(zip(SHAPE_KEYS, list(v)))
 
 
 def set_hop_shape(g: Gauge, precision, recon, nrhs, shape: dict):

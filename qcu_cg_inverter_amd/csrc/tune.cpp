@@ -24,6 +24,26 @@ static const char* cache_file() {
   return (p && *p) ? p : nullptr;
 }
 
+// The checks a launch shape must pass (qcuSetHopShape and the cache file): null when the
+// shape is valid for `precision`, else the reason.  Shapes outside them would divide by
+// zero in the launcher or launch blocks the kernels do not expect.
+static const char* shape_error(const Tune& t, int precision) {
+  if (!(t.lds >= 0 && t.lds <= 2)) return "lds must be 0, 1 or 2";
+  if (t.lds && !(t.block == 64 || t.block == 128 || t.block == 256))
+    return "the LDS kernel takes 1, 2 or 4 waves per block";
+  if (!(t.block == 64 || t.block == 128 || t.block == 192 || t.block == 256)) return "block must be 64, 128, 192 or 256";
+  if (!(t.rpw == 1 || t.rpw == 2 || t.rpw == 4)) return "right-hand sides per wave must be 1, 2 or 4";
+  if (!(t.spt >= 1 && t.spt <= 64)) return "site blocks per block must be 1..64";
+  if (!(t.cached == 0 || t.cached == 1)) return "load policy must be 0 (streaming) or 1 (cached)";
+  if (!(t.rb >= 1 && t.block % (64 * t.rb) == 0)) return "rb must divide block into 64-thread groups";
+  if (!(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1))) return "rpt must be 1..4 (1 for FP64)";
+  if (!((t.xcd == 0 || t.xcd == 1 || (t.lds && (t.xcd == 2 || t.xcd == 4))) && (t.gm == 0 || t.gm == 1)))
+    return "xcd is 0 or 1 (2 or 4: XCD column groups, LDS kernel only); group order is 0 or 1";
+  if (!(t.tz >= 0 && t.tt >= 0 && t.ty >= 0 && (t.tz == 0) == (t.tt == 0)))
+    return "tile extents must be non-negative, tz and tt both 0 or both positive";
+  return nullptr;
+}
+
 static bool cache_loaded = false;
 
 void load_tune_cache() {
@@ -37,8 +57,11 @@ void load_tune_cache() {
   while (fgets(line, sizeof line, f)) {
     Tune t;
     double ms;
+    int prec = -1;
+    // lines that do not parse, or whose shape qcuSetHopShape would refuse, are skipped
     if (sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %d %d %d %lf", key, &t.block, &t.rb, &t.rpt, &t.xcd, &t.tz,
-               &t.tt, &t.gm, &t.ty, &t.rpw, &t.spt, &t.cached, &t.lds, &ms) == 14)
+               &t.tt, &t.gm, &t.ty, &t.rpw, &t.spt, &t.cached, &t.lds, &ms) == 14 &&
+        sscanf(key, "hop:%d:", &prec) == 1 && prec >= 0 && prec <= 2 && !shape_error(t, prec))
       ctx().tune[key] = t;
   }
   fclose(f);
@@ -297,6 +320,25 @@ extern "C" int qcuTuneHop(qcuGauge h, int precision, int recon, int nrhs, int* s
   }
 }
 
+extern "C" int qcuGetHopShape(qcuGauge h, int precision, int recon, int nrhs, int* shape) {
+  using namespace qcu;
+  try {
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    ensure_init();
+    QREQ(h && shape, "qcuGetHopShape: NULL argument");
+    QREQ(precision >= 0 && precision <= 2, "bad precision");
+    QREQ(recon == 12 || recon == 18, "recon must be 12 or 18");
+    QREQ(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs out of range");
+    const Tune t = tuned_hop(precision, recon, reinterpret_cast<GaugeField*>(h)->geo.vh, nrhs);
+    const int v[12] = {t.block, t.rb, t.rpt, t.xcd, t.tz, t.tt, t.gm, t.ty, t.rpw, t.spt, t.cached, t.lds};
+    for (int k = 0; k < 12; ++k) shape[k] = v[k];
+    return QCU_SUCCESS;
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    return e.code;
+  }
+}
+
 extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, const int* shape) {
   using namespace qcu;
   try {
@@ -319,18 +361,7 @@ extern "C" int qcuSetHopShape(qcuGauge h, int precision, int recon, int nrhs, co
     t.spt = shape[9];
     t.cached = shape[10];
     t.lds = shape[11];
-    QREQ(t.lds >= 0 && t.lds <= 2, "lds must be 0, 1 or 2");
-    QREQ(!t.lds || t.block == 64 || t.block == 128 || t.block == 256, "the LDS kernel takes 1, 2 or 4 waves per block");
-    QREQ(t.block == 64 || t.block == 128 || t.block == 192 || t.block == 256, "block must be 64, 128, 192 or 256");
-    QREQ(t.rpw == 1 || t.rpw == 2 || t.rpw == 4, "right-hand sides per wave must be 1, 2 or 4");
-    QREQ(t.spt >= 1 && t.spt <= 64, "site blocks per block must be 1..64");
-    QREQ(t.cached == 0 || t.cached == 1, "load policy must be 0 (streaming) or 1 (cached)");
-    QREQ(t.rb >= 1 && t.block % (64 * t.rb) == 0, "rb must divide block into 64-thread groups");
-    QREQ(t.rpt >= 1 && t.rpt <= 4 && (precision != F64 || t.rpt == 1), "rpt must be 1..4 (1 for FP64)");
-    QREQ((t.xcd == 0 || t.xcd == 1 || (t.lds && (t.xcd == 2 || t.xcd == 4))) && (t.gm == 0 || t.gm == 1),
-         "xcd is 0 or 1 (2 or 4: XCD column groups, LDS kernel only); group order is 0 or 1");
-    QREQ(t.tz >= 0 && t.tt >= 0 && t.ty >= 0 && (t.tz == 0) == (t.tt == 0),
-         "tile extents must be non-negative, tz and tt both 0 or both positive");
+    if (const char* why = shape_error(t, precision)) throw Error(QCU_ERROR_INVALID_VALUE, why);
     GaugeField& g = *reinterpret_cast<GaugeField*>(h);
     set_tuned_hop(precision, recon, g.geo.vh, nrhs, t);
     return QCU_SUCCESS;

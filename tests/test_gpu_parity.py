@@ -513,6 +513,52 @@ def test_hop_shapes_keep_bits(q, prec):
             assert np.array_equal(outs[r], singles[r]), (sh, r)
 
 
+def test_tune_cache_file_skips_invalid_shapes(tmp_path):
+    """A QCU_TUNE_CACHE file is read by a fresh process: a valid line sets its shape, a
+    line with a shape qcuSetHopShape would refuse (block 100; a load policy of 7 after a
+    valid line of the same key) is skipped, and hops on the affected keys still run and
+    match the oracle."""
+    import os
+    import subprocess
+    import sys
+    import textwrap
+
+    vh = 8 * 8 * 8 * 8 // 2
+    cache = tmp_path / "tune.cache"
+    cache.write_text(f"hop:1:12:{vh}:1 64 1 1 0 0 0 0 0 1 1 1 0 0.1\n"
+                     f"hop:1:18:{vh}:1 100 5 1 0 0 0 0 0 1 1 0 0 0.1\n"
+                     f"hop:2:12:{vh}:1 128 1 1 1 0 0 0 0 1 1 0 0 0.1\n"
+                     f"hop:2:12:{vh}:1 128 1 1 1 0 0 0 0 1 1 7 0 0.1\n"
+                     f"not a shape line\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = textwrap.dedent("""
+        import sys
+        sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+        import numpy as np
+        import oracle as orc
+        import qcu_cg_inverter_amd as q
+        q.init(0)
+        L = (8, 8, 8, 8)
+        g = q.Gauge.random(L, seed=4, eps=0.3)
+        a = q.get_hop_shape(g, 1, 12, 1)
+        b = q.get_hop_shape(g, 1, 18, 1)
+        c = q.get_hop_shape(g, 2, 12, 1)
+        assert (a["block"], a["cached"]) == (64, 1), a
+        assert b["block"] != 100 and b["rb"] != 5, b
+        assert (c["block"], c["xcd"], c["cached"]) == (128, 1, 0), c
+        gref = g.to_numpy()
+        v = orc.source_gauss(L, 6)
+        for prec, recon, tol in ((1, 12, 1e-6), (1, 18, 1e-6), (2, 12, 3e-3)):
+            out = q.hop(g, v[1], 0, dagger=0, t_boundary=-1, recon=recon, precision=prec)
+            ref = orc.hop(v[1], gref, L, 0, s=1, tbc=-1.0, recon=recon)
+            assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol, (prec, recon)
+        print("cache ok")
+    """).format(root=root, tests=os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, QCU_TUNE_CACHE=str(cache))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "cache ok" in r.stdout, r.stdout + r.stderr
+
+
 @pytest.mark.parametrize("csw", [0.0, 1.0])
 def test_composites_match_oracle(q, csw):
     """The reference's full-lattice compositions (cloverDslashOneRound, its
