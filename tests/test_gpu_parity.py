@@ -482,6 +482,30 @@ def test_batch12_clover_mixed(q, shape):
         assert np.array_equal(xa, xs[i]) and sa.iterations == sts[i].iterations
 
 
+@pytest.mark.parametrize("shape", [None, (256, 1, 1, 0, 8, 8, 0, 0, 1, 1, 1, 1), (64, 1, 1, 1, 0, 0, 0, 0, 1, 3, 0, 2),
+                                   (256, 4, 3, 1, 0, 0, 0), (128, 1, 1, 0, 0, 0, 0, 0, 4, 1, 1)])
+def test_batch12_clover_fp16(q, shape):
+    """The same 12-point-source clover batch with the FP16 + per-site-norm inner loop,
+    whose clover epilogues multiply in packed halves (the clover from LDS or from
+    memory, several right-hand sides per thread or per wave): every right-hand side
+    converges to 1e-10 by the oracle and equals its own solve alone bit for bit."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=33, eps=0.25)
+    if shape is not None:
+        q.set_hop_shape(g, 2, 12, 12, dict(zip(q.SHAPE_KEYS, shape)))
+    op = q.Operator(0.122, 1.0, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=2)
+    bs = [orc.source_point(L, (0, 0, 0, 0), s, c) for s in range(4) for c in range(3)]
+    xs, sts = q.solve_batch(g, op, bs, sp)
+    gref = g.to_numpy()
+    A = orc.clover_fields(gref, L, 0.122, 1.0, recon=12)[0]
+    for i in (0, 7, 11):
+        r = bs[i] - orc.full_op(xs[i], gref, L, 0.122, A=A, s=1, tbc=-1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(bs[i]) <= 1.01e-10
+        xa, sa = q.solve(g, op, bs[i], sp)
+        assert np.array_equal(xa, xs[i]) and sa.iterations == sts[i].iterations
+
+
 @pytest.mark.parametrize("prec", [0, 1, 2])
 def test_hop_shapes_keep_bits(q, prec):
     """Every launch shape of a 12-RHS hop -- block size, wave groups sharing sites,
