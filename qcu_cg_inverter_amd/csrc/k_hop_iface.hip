@@ -16,22 +16,24 @@
 // chunks -- one wave per SIMD if a wave did all eight legs -- so splitting the legs
 // over four waves is what lets the memory latency of one wave overlap another's.
 //
-// The records a wave needs for one leg (64 neighbour spinors of 192 B and 64 links of
-// which 96 B are read) are gathered as consecutive 16-byte words -- lane l loads word
-// f = 64 k + l of the list, record f / W, word f % W -- so each load instruction reads
-// a contiguous run wherever the neighbours of consecutive sites are consecutive (all
-// but the lattice-edge rows).  All 18 loads of a leg are issued before the words go
-// into the wave's LDS tiles, padded to an odd number of 16-byte words per record so
-// that lanes reading their own record hit distinct banks.  The output leaves through
-// LDS the same way: the block stores 768 consecutive words.
+// The 64 neighbour spinors of 192 B a wave needs for one leg are gathered as
+// consecutive 16-byte words -- lane l loads word f = 64 k + l of the list, record
+// f / 12, word f % 12 -- so each load instruction reads a contiguous run wherever the
+// neighbours of consecutive sites are consecutive (all but the lattice-edge rows), and
+// the words reach their lanes through the wave's LDS tile, padded to 13 words per record
+// so that lanes reading their own record hit distinct banks.  Each lane loads the 96 B
+// of its own link's rows 0-1 directly: that leaves the block 52 KB of LDS, three blocks
+// per CU instead of two with a link tile as well (20.9 against 21.4 us per 16^3x32
+// call; loading the spinors directly too took 22.2 us).  The output leaves through LDS
+// the same way: the block stores 768 consecutive words (each wave storing its own spin
+// row's words took 21.6 us).
 #include "k_hop.inc"
 
 namespace qcu {
 
 constexpr int kIfaceSpPad = 13;  // LDS words per spinor record (12 + 1)
-constexpr int kIfaceLkPad = 7;   // LDS words per link record (6 + 1)
-// one wave's LDS region: its spinor and link tiles (20 KB), later its legs' link products
-constexpr int kIfaceWave = 64 * (kIfaceSpPad + kIfaceLkPad);
+// one wave's LDS region: its spinor tile (13 KB), later its legs' link products
+constexpr int kIfaceWave = 64 * kIfaceSpPad;
 
 // a wave's LDS accesses are processed in order; this keeps the compiler from moving
 // the wave's own loads and stores across the point where lanes trade records
@@ -53,41 +55,31 @@ __device__ __forceinline__ void iface_leg_t(const double2* __restrict__ in, cons
   const int lpar = FWD ? parity : 1 - parity;
   const double2* gb = gauge + (long)(MU * 2 + lpar) * vh * 9;
   double2* TS = W;
-  double2* TL = W + 64 * kIfaceSpPad;
-  double2 sw[12], lw[6];
+  double2 sw[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
     const int f = k * 64 + lane, j = f / 12, w = f - 12 * j;
     sw[k] = in[(long)from_lane(sp, j) * 12 + w];
   }
+  // the lane's own link rows 0-1, six 16-byte words of one record
+  cx<double> psi[12], u[9];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int f = k * 64 + lane, j = f / 6, w = f - 6 * j;
-    lw[k] = gb[(long)from_lane(lk, j) * 9 + w];
+  for (int e = 0; e < 6; ++e) {
+    const double2 q = gb[(long)lk * 9 + e];
+    u[e] = {q.x, q.y};
   }
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
     const int f = k * 64 + lane, j = f / 12, w = f - 12 * j;
     TS[j * kIfaceSpPad + w] = sw[k];
   }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int f = k * 64 + lane, j = f / 6, w = f - 6 * j;
-    TL[j * kIfaceLkPad + w] = lw[k];
-  }
   wave_sync();
-  cx<double> psi[12], u[9];
 #pragma unroll
   for (int w = 0; w < 12; ++w) {
     const double2 q = TS[lane * kIfaceSpPad + w];
     psi[w] = {q.x, q.y};
   }
-#pragma unroll
-  for (int e = 0; e < 6; ++e) {
-    const double2 q = TL[lane * kIfaceLkPad + e];
-    u[e] = {q.x, q.y};
-  }
-  wave_sync();  // the tiles are free again
+  wave_sync();  // the tile is free again
   recon_row2(u);
   hop_dir_t<MU, FWD, DAG>(u, psi, bs, t);
 }
@@ -149,7 +141,7 @@ __device__ __forceinline__ void iface_row(const double2* S, int lane, cx<double>
 // block b covers half sites [64 b, 64 b + 64); 256 threads, wave w = direction w, then
 // spin row w
 template <int DAG>
-__global__ void __launch_bounds__(256, 2) hop_iface_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+__global__ void __launch_bounds__(256, 3) hop_iface_kernel(const double2* __restrict__ in, double2* __restrict__ out,
                                                           const double2* __restrict__ gauge, const Geom g,
                                                           int parity, double tbc, double coef) {
   __shared__ double2 S[4 * kIfaceWave];
