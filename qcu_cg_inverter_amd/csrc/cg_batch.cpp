@@ -279,11 +279,12 @@ void CgBatch::harvest() {
       if (!std::isfinite(s.ss)) brk[r] = 1;
     }
   }
-  // top down, so the slot moved into a finished one has already been dealt with
-  for (int r = n - 1; r >= 0; --r) {
-    if (brk[r]) finish(r, true);
-    else if (upd[r] && (hsc_[r].converged || cap[r])) finish(r, false);
+  std::vector<char> done(n, 0);
+  for (int r = 0; r < n; ++r) {
+    if (brk[r]) done[r] = 2;
+    else if (upd[r] && (hsc_[r].converged || cap[r])) done[r] = 1;
   }
+  finish_slots(done);
   if (n_ > 0) QHIP(hipMemcpyAsync(dsc_, hsc_, sizeof(CgScal) * n_, hipMemcpyHostToDevice, st_));
 }
 
@@ -335,55 +336,119 @@ const void* CgBatch::source_aos(const Slot& sl) {
   return aos_.p;
 }
 
-// x_o, true residual, solution output and stats for slot r, then compaction.  Uses
-// the slot's bh_, shi_ and t64_ as scratch (b^ is no longer needed).
-void CgBatch::finish(int r, bool breakdown) {
-  Slot& sl = slots_[r];
-  const double kap = sl.spec.kappa;
-  flush_x(r);
+// |v|^2 of slots [lo, hi] of field v into out[0 .. hi - lo] (enqueued; read after a
+// synchronisation); the per-slot sums are those of norm2_slot
+void CgBatch::norm2_run(Vec v, int lo, int hi, double* out) {
+  const int m = hi - lo + 1;
+  double* part = part_ss_ + nw_ * cap_;  // spare partials region
+  launch_norm2(F64, at(v, lo), vh_, m, part, st_);
+  launch_sum_partials(part, nw_, m, sums_, st_);
+  QHIP(hipMemcpyAsync(out, sums_, sizeof(double) * m, hipMemcpyDeviceToHost, st_));
+}
+
+// x_o = A_oo^-1 (b_o + kappa D_oe x_e) and the true residual |b - M x|^2 of slots
+// [lo, hi], each with its own kappa, clover and source, the FP64 operators batched over
+// the run (a batched hop gives every slot the bits of its own launch).  Uses the slots'
+// bh_, shi_ and t64_ as scratch (b^ is no longer needed).
+void CgBatch::finish_run(int lo, int hi, std::vector<double>& nr) {
+  const int m = hi - lo + 1;
+  for (int r = lo; r <= hi; ++r) flush_x(r);
   if (prec_ != F64) {
-    launch_spinor_convert(xprec_, at(X_, r), F64, at(xe_, r), vh_, 1, 1, st_);
-    launch_zero(xprec_, at(X_, r), vh_, 1, st_);
+    launch_spinor_convert(xprec_, at(X_, lo), F64, at(xe_, lo), vh_, m, 1, st_);
+    launch_zero(xprec_, at(X_, lo), vh_, m, st_);
   }
-  const char* b = static_cast<const char*>(source_aos(sl));
-  Vec xo = at(shi_, r), tmp = at(bh_, r), bq = at(t64_, r);
-  launch_spinor_from_aos(F64, b + vh_ * 192, 0, bq, vh_, 1, st_);  // b_o
+  const void* none[kMaxRhs];
+  const void* a_o[kMaxRhs];
+  const void* a_e[kMaxRhs];
+  double kap[kMaxRhs], mkap[kMaxRhs];
+  for (int j = 0; j < m; ++j) {
+    const Slot& sl = slots_[lo + j];
+    none[j] = nullptr;
+    a_o[j] = clover_ ? sl.cf->a(F64, 1) : nullptr;
+    a_e[j] = clover_ ? sl.cf->a(F64, 0) : nullptr;
+    kap[j] = sl.spec.kappa;
+    mkap[j] = -sl.spec.kappa;
+  }
+  Vec xo = at(shi_, lo), tmp = at(bh_, lo), bq = at(t64_, lo);
+  for (int r = lo; r <= hi; ++r)  // b_o
+    launch_spinor_from_aos(F64, static_cast<const char*>(source_aos(slots_[r])) + vh_ * 192, 0, at(t64_, r), vh_,
+                           1, st_);
   // x_o = A_oo^-1 (b_o + kappa D_oe x_e)
   if (clover_) {
-    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), tmp, bq, nullptr, kap, 1, nullptr, st_);
-    launch_clover_apply(F64, sl.cf->ainv(F64, 1), tmp, xo, vh_, 1, st_);
+    hop_v(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, lo), tmp, bq, none, kap, m, nullptr, st_);
+    for (int r = lo; r <= hi; ++r)
+      launch_clover_apply(F64, slots_[r].cf->ainv(F64, 1), at(bh_, r), at(shi_, r), vh_, 1, st_);
   } else {
-    hop(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), xo, bq, nullptr, kap, 1, nullptr, st_);
+    hop_v(g_, F64, recon_, EPI_XPAY, 1, 0, (double)tbc_, at(xe_, lo), xo, bq, none, kap, m, nullptr, st_);
   }
   // r = b - M x with M = A - kappa D, odd then even parity
-  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 1, 0, (double)tbc_, at(xe_, r), tmp, xo,
-      clover_ ? sl.cf->a(F64, 1) : nullptr, -kap, 1, nullptr, st_);
-  launch_axpby(F64, 1.0, bq, -1.0, tmp, vh_, 1, st_);
-  double nr = norm2_slot(bh_, r);  // |r_o|^2
-  launch_spinor_from_aos(F64, b, 0, bq, vh_, 1, st_);  // b_e
-  hop(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 0, 0, (double)tbc_, xo, tmp, at(xe_, r),
-      clover_ ? sl.cf->a(F64, 0) : nullptr, -kap, 1, nullptr, st_);
-  launch_axpby(F64, 1.0, bq, -1.0, tmp, vh_, 1, st_);
-  if (sl.spec.x) {
-    launch_spinor_to_aos(F64, at(xe_, r), sl.spec.x, 0, vh_, 1, st_);
-    launch_spinor_to_aos(F64, xo, static_cast<char*>(sl.spec.x) + vh_ * 192, 0, vh_, 1, st_);
+  std::vector<double> ro(m), re(m);
+  hop_v(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 1, 0, (double)tbc_, at(xe_, lo), tmp, xo,
+        clover_ ? a_o : none, mkap, m, nullptr, st_);
+  launch_axpby(F64, 1.0, bq, -1.0, tmp, vh_, m, st_);
+  norm2_run(bh_, lo, hi, ro.data());
+  for (int r = lo; r <= hi; ++r)  // b_e
+    launch_spinor_from_aos(F64, source_aos(slots_[r]), 0, at(t64_, r), vh_, 1, st_);
+  hop_v(g_, F64, recon_, clover_ ? EPI_CLOV_XPAY : EPI_XPAY, 0, 0, (double)tbc_, xo, tmp, at(xe_, lo),
+        clover_ ? a_e : none, mkap, m, nullptr, st_);
+  launch_axpby(F64, 1.0, bq, -1.0, tmp, vh_, m, st_);
+  for (int r = lo; r <= hi; ++r) {
+    const Slot& sl = slots_[r];
+    if (sl.spec.x) {
+      launch_spinor_to_aos(F64, at(xe_, r), sl.spec.x, 0, vh_, 1, st_);
+      launch_spinor_to_aos(F64, at(shi_, r), static_cast<char*>(sl.spec.x) + vh_ * 192, 0, vh_, 1, st_);
+    }
   }
+  norm2_run(bh_, lo, hi, re.data());
   QHIP(hipGetLastError());
-  nr = norm2_slot(bh_, r) + nr;  // |r_e|^2 + |r_o|^2
-  SlotResult res;
-  res.tag = sl.spec.tag;
-  res.stats.iterations = hsc_[r].iter;
-  res.stats.reliable_updates = sl.updates;
-  res.stats.breakdown = breakdown ? 1 : 0;
-  res.stats.converged = (!breakdown && hsc_[r].converged) ? 1 : 0;
-  res.stats.true_residual = sl.bnorm2 > 0 ? std::sqrt(nr / sl.bnorm2) : 0.0;
-  res.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
-  finished_.push_back(res);
-  const int last = n_ - 1;
-  if (r != last) move_slot(last, r);
-  slots_.pop_back();
-  n_--;
+  QHIP(hipStreamSynchronize(st_));
+  for (int r = lo; r <= hi; ++r) nr[r] = re[r - lo] + ro[r - lo];  // |r_e|^2 + |r_o|^2
+}
+
+void CgBatch::finish_slots(const std::vector<char>& done) {
+  const int n = n_;
+  bool any = false;
+  for (int r = 0; r < n; ++r) any = any || done[r];
+  if (!any) return;  // the captured chunks stay valid
+  std::vector<double> nr(n, 0.0);
+  for (int lo = 0; lo < n;) {
+    if (!done[lo]) {
+      ++lo;
+      continue;
+    }
+    int hi = lo;
+    while (hi + 1 < n && done[hi + 1]) ++hi;
+    finish_run(lo, hi, nr);
+    lo = hi + 1;
+  }
+  // stats, then compaction top down, so the slot moved into a finished one has
+  // already been dealt with
+  for (int r = n - 1; r >= 0; --r) {
+    if (!done[r]) continue;
+    const Slot& sl = slots_[r];
+    const bool breakdown = done[r] == 2;
+    SlotResult res;
+    res.tag = sl.spec.tag;
+    res.stats.iterations = hsc_[r].iter;
+    res.stats.reliable_updates = sl.updates;
+    res.stats.breakdown = breakdown ? 1 : 0;
+    res.stats.converged = (!breakdown && hsc_[r].converged) ? 1 : 0;
+    res.stats.true_residual = sl.bnorm2 > 0 ? std::sqrt(nr[r] / sl.bnorm2) : 0.0;
+    res.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
+    finished_.push_back(res);
+    const int last = n_ - 1;
+    if (r != last) move_slot(last, r);
+    slots_.pop_back();
+    n_--;
+  }
   clear_graphs();
+}
+
+// finish one slot (x_o, true residual, solution, stats, compaction)
+void CgBatch::finish(int r, bool breakdown) {
+  std::vector<char> done(n_, 0);
+  done[r] = breakdown ? 2 : 1;
+  finish_slots(done);
 }
 
 void CgBatch::move_slot(int from, int to) {

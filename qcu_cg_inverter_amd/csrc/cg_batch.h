@@ -77,6 +77,13 @@ class CgBatch {
   double reliable_update(int r);
   std::vector<double> reliable_updates(int lo, int hi, const std::vector<char>& upd);
   void finish(int r, bool breakdown);
+  // finish the slots marked in `done` (1 converged or capped, 2 broke down), their FP64
+  // work batched over contiguous runs of marked slots, then compact top down
+  void finish_slots(const std::vector<char>& done);
+  // x_o and |b - M x|^2 (into nr[r]) for slots [lo, hi], solutions written out
+  void finish_run(int lo, int hi, std::vector<double>& nr);
+  // |v|^2 of slots [lo, hi] of field v, in order (asynchronous: valid after a sync)
+  void norm2_run(Vec v, int lo, int hi, double* out);
   const void* source_aos(const Slot& sl);
   void move_slot(int from, int to);
   // slot r of a batch field; v must be the field's base (slot 0), never a slot view
