@@ -359,6 +359,34 @@ def test_batch_equals_alone(q, prec):
         assert sts[i].true_residual <= 1e-10
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_batch_slots_finish_apart(q, prec):
+    """Point and Gaussian sources interleaved in one batch, checked after every
+    iteration (chunk 1), so slots converge at different harvests and the ones that
+    finish together are not neighbours: each right-hand side still equals its own solve
+    alone bit for bit, with the same iteration count and its true residual by the
+    oracle."""
+    L = (8, 8, 4, 8)
+    g = q.Gauge.random(L, seed=8, eps=0.25)
+    kappa = 0.122
+    op = q.Operator(kappa, 1.0, -1, 12)
+    sp = q.default_solve_params(tol=1e-10, precision=prec, chunk=1)
+    bs = [orc.source_point(L, (0, 0, 0, 0), 0, 0), orc.source_gauss(L, 101),
+          orc.source_point(L, (3, 2, 1, 5), 2, 1), orc.source_gauss(L, 102),
+          orc.source_point(L, (7, 0, 3, 2), 3, 2)]
+    xs, sts = q.solve_batch(g, op, bs, sp)
+    its = [s.iterations for s in sts]
+    assert len(set(its)) >= 2, its
+    gref = g.to_numpy()
+    A = orc.clover_fields(gref, L, kappa, 1.0, recon=12)[0]
+    for i, b in enumerate(bs):
+        xa, sa = q.solve(g, op, b, sp)
+        assert np.array_equal(xa, xs[i]), (i, its)
+        assert sa.iterations == sts[i].iterations and sts[i].converged
+        r = b - orc.full_op(xs[i], gref, L, kappa, A=A, s=1, tbc=-1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-10
+
+
 def test_zero_source(q):
     """b = 0 gives x = 0 after no iterations with true residual 0, alone and in a batch
     beside nonzero sources, whose bits it leaves unchanged."""
