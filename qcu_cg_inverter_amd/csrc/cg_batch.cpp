@@ -125,63 +125,97 @@ void CgBatch::mhat(int prec, int dag, Vec in, Vec out, Vec tmp, int r0, int n, d
         at(in, r0), c2, k2, n, partials ? partials + (long)r0 * nw_ : nullptr, st_);
 }
 
-void CgBatch::add(const SlotSpec& spec) {
-  QREQ(n_ < cap_, "batch is full");
-  check_operator(spec);
-  const int r = n_;
-  Slot sl;
-  sl.spec = spec;
-  sl.updates = 0;
-  sl.cf = nullptr;
-  if (clover_) {
-    sl.cf = g_.clover(spec.kappa * spec.csw, F64);
-    if (prec_ != F64) g_.clover(spec.kappa * spec.csw, prec_);
-    QHIP(hipStreamSynchronize(ctx().stream));
+void CgBatch::add(const SlotSpec& spec) { add_slots(&spec, 1); }
+
+// The m new slots take [n_, n_ + m); their FP64 set-up runs batched over that run (a
+// batched hop gives every slot the bits of its own launch), with two synchronisations
+// in all.
+void CgBatch::add_slots(const SlotSpec* specs, int m) {
+  QREQ(m >= 1 && n_ + m <= cap_, "batch is full");
+  for (int j = 0; j < m; ++j) check_operator(specs[j]);
+  const int r0 = n_, r1 = n_ + m - 1;
+  for (int j = 0; j < m; ++j) {
+    Slot sl;
+    sl.spec = specs[j];
+    sl.updates = 0;
+    sl.cf = nullptr;
+    if (clover_) {
+      sl.cf = g_.clover(sl.spec.kappa * sl.spec.csw, F64);
+      if (prec_ != F64) g_.clover(sl.spec.kappa * sl.spec.csw, prec_);
+    }
+    if (sl.spec.max_iter <= 0) sl.spec.max_iter = (int)std::min<long>(20 * vh_, 1000000);
+    slots_.push_back(sl);
   }
-  if (sl.spec.max_iter <= 0) sl.spec.max_iter = (int)std::min<long>(20 * vh_, 1000000);
-  slots_.push_back(sl);
-  n_++;
-  Slot& s = slots_[r];
+  n_ += m;
+  if (clover_) QHIP(hipStreamSynchronize(ctx().stream));  // clover fields built there
   // b_e in shi_, b_o in t64_ (both scratch until the iteration starts)
-  const char* b = static_cast<const char*>(source_aos(s));
-  launch_spinor_from_aos(F64, b, 0, at(shi_, r), vh_, 1, st_);
-  launch_spinor_from_aos(F64, b + vh_ * 192, 0, at(t64_, r), vh_, 1, st_);
-  QHIP(hipGetLastError());
-  s.bnorm2 = norm2_slot(shi_, r) + norm2_slot(t64_, r);
-  if (!std::isfinite(s.bnorm2)) {
-    slots_.pop_back();
-    n_--;
-    throw Error(QCU_ERROR_NOT_FINITE, "source contains NaN or Inf");
+  for (int r = r0; r <= r1; ++r) {
+    const char* b = static_cast<const char*>(source_aos(slots_[r]));
+    launch_spinor_from_aos(F64, b, 0, at(shi_, r), vh_, 1, st_);
+    launch_spinor_from_aos(F64, b + vh_ * 192, 0, at(t64_, r), vh_, 1, st_);
   }
-  // b^ = b_e + kappa D_eo A_oo^-1 b_o
-  const double kap = spec.kappa;
-  if (clover_) launch_clover_apply(F64, s.cf->ainv(F64, 1), at(t64_, r), at(t64_, r), vh_, 1, st_);  // in place
-  hop(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(t64_, r), at(bh_, r), at(shi_, r), nullptr, kap, 1, nullptr,
-      st_);
-  launch_zero(F64, at(xe_, r), vh_, 1, st_);
-  QHIP(hipMemcpyAsync(at(shi_, r).p, at(bh_, r).p, fb_, hipMemcpyDeviceToDevice, st_));
+  QHIP(hipGetLastError());
+  std::vector<double> ne(m), no(m);
+  norm2_run(shi_, r0, r1, ne.data());
+  norm2_run(t64_, r0, r1, no.data());
+  QHIP(hipStreamSynchronize(st_));
+  for (int j = 0; j < m; ++j) {
+    slots_[r0 + j].bnorm2 = ne[j] + no[j];
+    if (!std::isfinite(slots_[r0 + j].bnorm2)) {
+      slots_.resize(r0);
+      n_ = r0;
+      throw Error(QCU_ERROR_NOT_FINITE, "source contains NaN or Inf");
+    }
+  }
+  // b^ = b_e + kappa D_eo A_oo^-1 b_o, A_oo^-1 b_o in place over runs sharing a clover
+  const void* none[kMaxRhs];
+  double kap[kMaxRhs];
+  for (int j = 0; j < m; ++j) {
+    none[j] = nullptr;
+    kap[j] = slots_[r0 + j].spec.kappa;
+  }
+  if (clover_) {
+    for (int lo = r0; lo <= r1;) {
+      int hi = lo;
+      while (hi < r1 && slots_[hi + 1].cf == slots_[lo].cf) ++hi;
+      launch_clover_apply(F64, slots_[lo].cf->ainv(F64, 1), at(t64_, lo), at(t64_, lo), vh_, hi - lo + 1, st_);
+      lo = hi + 1;
+    }
+  }
+  hop_v(g_, F64, recon_, EPI_XPAY, 0, 0, (double)tbc_, at(t64_, r0), at(bh_, r0), at(shi_, r0), none, kap, m,
+        nullptr, st_);
+  launch_zero(F64, at(xe_, r0), vh_, m, st_);
+  for (int r = r0; r <= r1; ++r)
+    QHIP(hipMemcpyAsync(at(shi_, r).p, at(bh_, r).p, fb_, hipMemcpyDeviceToDevice, st_));
   if (prec_ != F64) {
-    launch_spinor_convert(F64, at(shi_, r), prec_, at(S_, r), vh_, 1, 0, st_);
-    launch_zero(xprec_, at(X_, r), vh_, 1, st_);
+    launch_spinor_convert(F64, at(shi_, r0), prec_, at(S_, r0), vh_, m, 0, st_);
+    launch_zero(xprec_, at(X_, r0), vh_, m, st_);
   }
-  launch_zero(prec_, at(P_, r), vh_, 1, st_);  // scratch may hold stale, non-finite bits
+  launch_zero(prec_, at(P_, r0), vh_, m, st_);  // scratch may hold stale, non-finite bits
   QHIP(hipGetLastError());
-  double ss0 = norm2_slot(shi_, r);
-  s.t0 = std::chrono::steady_clock::now();
-  CgScal& c = hsc_[r];
-  memset(&c, 0, sizeof c);
-  c.ss = ss0;
-  c.ss_ref = ss0;
-  c.target2 = spec.tol * spec.tol * s.bnorm2;
-  c.first = 1;
-  c.converged = ss0 <= c.target2;
-  c.active = 1;
-  clear_graphs();
-  if (c.converged) {
-    finish(r, false);
-  } else {
-    QHIP(hipMemcpyAsync(dsc_ + r, hsc_ + r, sizeof(CgScal), hipMemcpyHostToDevice, st_));
+  std::vector<double> ss0(m);
+  norm2_run(shi_, r0, r1, ss0.data());
+  QHIP(hipStreamSynchronize(st_));
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<char> done(n_, 0);
+  bool any = false;
+  for (int j = 0; j < m; ++j) {
+    Slot& sl = slots_[r0 + j];
+    sl.t0 = t0;
+    CgScal& c = hsc_[r0 + j];
+    memset(&c, 0, sizeof c);
+    c.ss = ss0[j];
+    c.ss_ref = ss0[j];
+    c.target2 = sl.spec.tol * sl.spec.tol * sl.bnorm2;
+    c.first = 1;
+    c.converged = ss0[j] <= c.target2;
+    c.active = 1;
+    done[r0 + j] = c.converged ? 1 : 0;
+    any = any || c.converged;
   }
+  clear_graphs();
+  if (any) finish_slots(done);  // compaction moves slots within [r0, n_) only
+  if (n_ > r0) QHIP(hipMemcpyAsync(dsc_ + r0, hsc_ + r0, sizeof(CgScal) * (n_ - r0), hipMemcpyHostToDevice, st_));
 }
 
 void CgBatch::launch_chunk() {

@@ -57,6 +57,9 @@ class CgBatch {
 
   // prepare a right-hand side in a free slot (synchronises the batch stream)
   void add(const SlotSpec& s);
+  // prepare m right-hand sides in free slots, their set-up batched; a non-finite
+  // source rejects all m and leaves the batch as it was
+  void add_slots(const SlotSpec* specs, int m);
   // enqueue one chunk of iterations for the active slots (asynchronous)
   void launch_chunk();
   // after the chunk: reliable updates, convergence, finished slots (synchronises)

@@ -130,8 +130,9 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
   batch.take_finished();
   std::vector<SlotResult> res;
   try {
+    std::vector<SlotSpec> specs(nrhs);
     for (int r = 0; r < nrhs; ++r) {
-      SlotSpec s;
+      SlotSpec& s = specs[r];
       s.b = b[r];
       s.seed = 0;
       s.x = x[r];
@@ -140,8 +141,8 @@ void solve_batch(GaugeField& g, const qcuOperator& op, const qcuSolveParams& spi
       s.tol = sp.tol;
       s.max_iter = sp.max_iter;
       s.tag = (uint64_t)r;
-      batch.add(s);
     }
+    batch.add_slots(specs.data(), nrhs);
     while (batch.active() > 0) {
       batch.launch_chunk();
       batch.harvest();
