@@ -191,43 +191,62 @@ Server::Admit Server::admit_one(uint64_t id, Request& r, bool allow_alloc) {
       release_idle_batches(be);  // idle batches of other gauges may hold the memory
       if (used + cc > budget) return ADMIT_NO_MEMORY;
     }
+    // as many of the request's right-hand sides as the batch has free slots, set up
+    // together
+    const int m = std::min(r.spec.nrhs - r.admitted, be->batch->free_slots());
     if (clover) {
       used += cc;
       if (used > peak) peak = used;
-      clover_refs[{g, ck}]++;
+      clover_refs[{g, ck}] += m;
     }
-    const int k = r.admitted;
-    SlotSpec s;
-    s.b = r.b.empty() ? nullptr : r.b[k];
-    s.seed = r.spec.seed + (uint64_t)k;
-    s.x = r.x.empty() ? nullptr : r.x[k];
-    s.kappa = r.spec.kappa;
-    s.csw = r.spec.csw;
-    s.tol = r.spec.tol;
-    s.max_iter = r.spec.max_iter;
-    s.tag = next_tag++;
-    bool rejected = false;
+    std::vector<SlotSpec> specs(m);
+    for (int j = 0; j < m; ++j) {
+      const int k = r.admitted + j;
+      SlotSpec& s = specs[j];
+      s.b = r.b.empty() ? nullptr : r.b[k];
+      s.seed = r.spec.seed + (uint64_t)k;
+      s.x = r.x.empty() ? nullptr : r.x[k];
+      s.kappa = r.spec.kappa;
+      s.csw = r.spec.csw;
+      s.tol = r.spec.tol;
+      s.max_iter = r.spec.max_iter;
+      s.tag = next_tag++;
+    }
+    std::vector<char> added(m, 1);
     try {
-      be->batch->add(s);  // device work: the request lock is not held here
-      be->slot_req[s.tag] = {id, k};
+      be->batch->add_slots(specs.data(), m);  // device work: the request lock is not held here
     } catch (const Error& e) {
-      // this right-hand side cannot be solved (e.g. a non-finite source): it fails
-      // alone, the server and the other requests go on
       if (e.code == QCU_ERROR_HIP || e.code == QCU_ERROR_OUT_OF_MEMORY) throw;
-      if (clover) release_clover(g, ck, false);
-      rejected = true;
+      // a right-hand side that cannot be solved (e.g. a non-finite source) fails
+      // alone, the server and the other requests go on: add the group one by one
+      for (int j = 0; j < m; ++j) {
+        try {
+          be->batch->add(specs[j]);
+        } catch (const Error& e1) {
+          if (e1.code == QCU_ERROR_HIP || e1.code == QCU_ERROR_OUT_OF_MEMORY) throw;
+          added[j] = 0;
+          if (clover) release_clover(g, ck, false);
+        }
+      }
     }
-    if (clover && !rejected) {
-      auto c = g->clovers.find(ck);  // built by add()
-      if (c != g->clovers.end()) c->second->server_refs++;
+    int nadded = 0;
+    for (int j = 0; j < m; ++j)
+      if (added[j]) {
+        be->slot_req[specs[j].tag] = {id, r.admitted + j};
+        nadded++;
+      }
+    if (clover && nadded > 0) {
+      auto c = g->clovers.find(ck);  // built by add_slots()
+      if (c != g->clovers.end()) c->second->server_refs += nadded;
     }
     std::lock_guard<std::mutex> lk(mu);
     if (!r.started) {
       r.started = true;
       r.first_admit = Clock::now();
     }
-    r.admitted++;
-    if (rejected) complete_rhs(r, false, 0, std::numeric_limits<double>::infinity());
+    r.admitted += m;
+    for (int j = 0; j < m; ++j)
+      if (!added[j]) complete_rhs(r, false, 0, std::numeric_limits<double>::infinity());
   }
   return ADMIT_ALL;
 }

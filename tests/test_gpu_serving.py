@@ -128,6 +128,36 @@ def test_server_bad_request_fails_alone(q):
     s.destroy()
 
 
+def test_server_bad_rhs_of_a_group_fails_alone(q):
+    """A request's right-hand sides are set up together; when one of them has a
+    non-finite source it fails alone: the request ends not converged with an infinite
+    residual, and its other right-hand sides are solved, with the bits they get in a
+    request of their own and their true residual by the oracle."""
+    L = (8, 8, 8, 8)
+    g = q.Gauge.random(L, seed=45, eps=0.25)
+    vh = q.half_volume(L)
+    good = [orc.source_gauss(L, 80), orc.source_gauss(L, 81)]
+    bad = orc.source_gauss(L, 82)
+    bad[0, 3, 1, 2] = np.inf
+    s = q.Server(precision=1, max_batch=8, chunk=4)
+    bs = [q.DeviceBuffer.from_numpy(good[0]), q.DeviceBuffer.from_numpy(bad), q.DeviceBuffer.from_numpy(good[1])]
+    rid, xs = run_request(q, s, g, L, 0.121, 1.0, 1e-10, bs)
+    st = s.wait(rid, raise_on_error=False)
+    assert st.done and not st.converged and not np.isfinite(st.true_residual)
+    ref_bs = [q.DeviceBuffer.from_numpy(b) for b in good]
+    rid2, xs2 = run_request(q, s, g, L, 0.121, 1.0, 1e-10, ref_bs)
+    st2 = s.wait(rid2)
+    assert st2.converged and st2.true_residual <= 1e-10
+    s.destroy()
+    gref = g.to_numpy()
+    A = orc.clover_fields(gref, L, 0.121, 1.0, recon=12)[0]
+    for i, j in ((0, 0), (2, 1)):
+        x = xs[i].to_numpy((2, vh, 4, 3))
+        assert np.array_equal(x, xs2[j].to_numpy((2, vh, 4, 3))), i
+        r = good[j] - orc.full_op(x, gref, L, 0.121, A=A, s=1, tbc=-1.0, recon=12)
+        assert np.linalg.norm(r) / np.linalg.norm(good[j]) <= 1.01e-10
+
+
 def test_server_stalled_request_not_starved(q):
     """A request that needs memory held by a busy batch of another gauge gets in while
     a backlog for that batch is still queued: after a short grace nothing behind it is
