@@ -478,13 +478,6 @@ void CgBatch::finish_slots(const std::vector<char>& done) {
   clear_graphs();
 }
 
-// finish one slot (x_o, true residual, solution, stats, compaction)
-void CgBatch::finish(int r, bool breakdown) {
-  std::vector<char> done(n_, 0);
-  done[r] = breakdown ? 2 : 1;
-  finish_slots(done);
-}
-
 void CgBatch::move_slot(int from, int to) {
   Vec keep[] = {bh_, xe_, shi_, P_};
   for (const Vec& v : keep)

@@ -79,7 +79,6 @@ class CgBatch {
   void flush_x(int r);
   double reliable_update(int r);
   std::vector<double> reliable_updates(int lo, int hi, const std::vector<char>& upd);
-  void finish(int r, bool breakdown);
   // finish the slots marked in `done` (1 converged or capped, 2 broke down), their FP64
   // work batched over contiguous runs of marked slots, then compact top down
   void finish_slots(const std::vector<char>& done);
